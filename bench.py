@@ -1,0 +1,201 @@
+"""Benchmark: logical Jaro-Winkler pair comparisons per second of the linkage
+hot path on one MI355X.
+
+Workload (default): plain linkage of 1,000,000 x 1,000,000 synthetic records,
+three string variables (first_name, last_name, street) plus an exact-match
+int32 house number. One step = the device stage of Comparison.fit from
+encoded, HBM-resident columns: all-pairs JW levels of the three string
+variables (unique values, expansion to record pairs, sort), the exact-match
+pairs, and the pattern merge into the 54-pattern histogram. value =
+n_A * n_B * 3 / step time.
+
+Prints one JSON line (rank 0) with the roofline of the comparison kernel and a
+CPU baseline (the oracle's C restatement on a bounded row sample).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fast-er_amd"))
+
+# MI355X peaks used for the roofline fractions.
+VALU_LANE_OPS_PEAK = 256 * 4 * 16 * 2.4e9  # 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 3.93e13 lane-ops/s
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1_000_000, help="records per side")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (it would shard by A-row "
+              "ranges, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import faster
+    from faster import _ops
+    from faster.comparison import exact_keys_link, fuzzy_keys_link
+    from faster.encode import group_exact_link, group_strings
+    from faster.synthetic import STRING_VARS, linkage_tables
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    n = args.n
+    log(f"generating {n} x {n} synthetic records")
+    t0 = time.time()
+    dfa, dfb, _ = linkage_tables(n, n, seed=args.seed)
+    t_gen = time.time() - t0
+    log(f"generated in {t_gen:.1f} s; encoding columns (host unique + packing)")
+    t0 = time.time()
+    groups = [(group_strings(dfa[v].to_numpy()), group_strings(dfb[v].to_numpy())) for v in STRING_VARS]
+    ex = group_exact_link(dfa["house_number"].to_numpy(), dfb["house_number"].to_numpy())
+    t_enc = time.time() - t0
+    t0 = time.time()
+    dgroups = [(a.to_device(), b.to_device()) for a, b in groups]
+    ga_ex, sb_ex, rb_ex = ex.to_device()
+    torch.cuda.synchronize()
+    t_h2d = time.time() - t0
+    log(f"encoded in {t_enc:.1f} s, uploaded in {t_h2d:.2f} s; unique per side: "
+        + ", ".join(f"{a.n_unique}/{b.n_unique}" for a, b in groups))
+
+    def step(stats=None):
+        keys = [fuzzy_keys_link(a, b, 0.1, 0.88, 0.94, stats) for a, b in dgroups]
+        keys.append(exact_keys_link(ga_ex, sb_ex, rb_ex, ex.n_groups, n))
+        ids, offs, n_union = _ops.pattern_merge(keys, [3, 3, 3, 2])
+        return keys, ids, offs, n_union
+
+    for w in range(args.warmup):
+        log(f"warmup step {w + 1}/{args.warmup}")
+        step()
+        torch.cuda.synchronize()
+    times = []
+    stats_all = []
+    for k in range(args.steps):
+        st = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        keys, ids, offs, n_union = step(st)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        stats_all.append(st)
+        log(f"step {k + 1}/{args.steps}: {dt * 1e3:.1f} ms")
+    ms = 1e3 * sum(times) / len(times)
+    comps = float(n) * float(n) * 3
+    value = comps / (ms / 1e3)
+
+    # Roofline of the comparison kernel: algorithmic lane-ops / kernel time.
+    kern_ms = np.mean([sum(s["kernel_ms"] for s in st) for st in stats_all])
+    alg_ops = np.mean([sum(s["alg_lane_ops"] for s in st) for st in stats_all])
+    achieved = alg_ops / (kern_ms / 1e3)
+    st0 = stats_all[-1]
+    counts = np.concatenate([[n * n - int(n_union)], np.diff(np.asarray(offs))[1:]]).astype(np.int64)
+    level_sizes = [int(k.numel()) for k in keys]
+
+    # EM on the histogram (reported, not part of value).
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    np.random.seed(0)
+    est = faster.Estimation(3, 1, counts)
+    est.fit(Tolerance=1e-6)
+    t_em = time.perf_counter() - t0
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(groups, n, args.cpu_seconds)
+
+    line = {
+        "metric": "JW pair-comparisons/s at 1M x 1M, 3 string vars (plain linkage)",
+        "value": value,
+        "unit": "comparisons/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded generator, lognormal lengths, 30% perturbed copies)",
+        "config": {"workload": f"plain linkage {n} x {n} records, 3 string vars + int32 house number, "
+                               "thresholds [0.88, 0.94], p = 0.1", "n_a": n, "n_b": n,
+                   "string_vars": list(STRING_VARS), "exact_vars": ["house_number"], "patterns": 54},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
+                     "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK, "traffic": None,
+                     "kernel": "k_jw_tiles (all-pairs JW, 3 vars)", "kernel_ms": kern_ms,
+                     "alg_lane_ops": alg_ops},
+        "cpu_baseline": cpu,
+        "extras": {
+            "pairs_kernel_per_var": [s["pairs_kernel"] for s in st0],
+            "pairs_pruned_per_var": [s["pairs_pruned"] for s in st0],
+            "alphabet": [s["alphabet"] for s in st0],
+            "kernel_comparisons_per_s": comps / (kern_ms / 1e3),
+            "level_list_sizes": level_sizes,
+            "union_pairs": int(n_union),
+            "counts_head": counts[:6].tolist(),
+            "em_ms": t_em * 1e3,
+            "em_iterations": est.Iterations,
+            "host_encode_s": t_enc,
+            "h2d_s": t_h2d,
+            "value_incl_host_prep": comps / (ms / 1e3 + t_enc + t_h2d),
+            "step_ms_all": [t * 1e3 for t in times],
+        },
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(groups, n, seconds):
+    """The oracle's C restatement on all host cores, on a contiguous slice of
+    unique A values of each string variable against all unique B values
+    (work per A value is uniform under the generator), scaled to
+    comparisons/s."""
+    import time as _t
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import faster_oracle as oracle
+
+    threads = oracle.n_threads()
+    # Calibrate on a small slice, then size the sample for ~`seconds`.
+    ga, gb = groups[0]
+    sub = lambda g, r: (g.chars, g.offsets[:r + 1])
+    t0 = _t.perf_counter()
+    r0 = max(threads, 8)
+    oracle.jaro_winkler_all_pairs(sub(ga, r0), (gb.chars, gb.offsets), threads=threads)
+    rate = r0 * gb.n_unique / (_t.perf_counter() - t0)
+    rows = max(threads, int(rate * seconds / (3 * gb.n_unique)))
+    log(f"CPU baseline: {threads} threads, ~{rate:.3g} pairs/s, sampling {rows} A values x 3 vars")
+    pairs = 0
+    t0 = _t.perf_counter()
+    for ga, gb in groups:
+        r = min(rows, ga.n_unique)
+        oracle.jaro_winkler_all_pairs(sub(ga, r), (gb.chars, gb.offsets), threads=threads)
+        pairs += r * gb.n_unique
+    dt = _t.perf_counter() - t0
+    return {"value": pairs / dt, "unit": "comparisons/s", "cores": threads, "kind": "port",
+            "sample": f"{rows} unique A values x all {groups[0][1].n_unique} unique B values per string var "
+                      f"(3 vars, {pairs:.3g} JW pairs, {dt:.1f} s); all-pairs level scan only (no expansion/merge)"}
+
+
+if __name__ == "__main__":
+    main()

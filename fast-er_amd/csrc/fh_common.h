@@ -1,0 +1,136 @@
+// Shared host/device helpers for libfaster_hip (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/faster_hip.h"
+
+namespace fh {
+
+// ---------------------------------------------------------------------------
+// Errors: every C-ABI entry point returns an int status and leaves a
+// thread-local message readable through fh_last_error().
+// ---------------------------------------------------------------------------
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define FH_HIP(expr)                                                          \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      ::fh::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr,      \
+                      hipGetErrorString(_e));                                 \
+      return FH_ERR_HIP;                                                      \
+    }                                                                         \
+  } while (0)
+
+#define FH_TRY(expr)                 \
+  do {                               \
+    int _s = (expr);                 \
+    if (_s != FH_OK) return _s;      \
+  } while (0)
+
+#define FH_REQUIRE(cond, ...)             \
+  do {                                    \
+    if (!(cond)) {                        \
+      ::fh::set_error(__VA_ARGS__);       \
+      return FH_ERR_ARG;                  \
+    }                                     \
+  } while (0)
+
+// Stream-ordered device scratch owned by one entry-point call. Memory comes
+// from HIP's stream-ordered pool and is returned on the same stream when the
+// object goes out of scope.
+class Scratch {
+ public:
+  explicit Scratch(hipStream_t s) : stream_(s) {}
+  ~Scratch() { release(); }
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+
+  // Allocates `bytes` (rounded up to 256) and returns FH_OK or an error.
+  template <typename T>
+  int alloc(T** out, size_t count) {
+    size_t bytes = ((count * sizeof(T)) + 255) & ~size_t(255);
+    if (bytes == 0) bytes = 256;
+    void* p = nullptr;
+    hipError_t e = hipMallocAsync(&p, bytes, stream_);
+    if (e != hipSuccess) {
+      set_error("hipMallocAsync(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+      return FH_ERR_HIP;
+    }
+    if (n_ >= kMax) {
+      (void)hipFreeAsync(p, stream_);
+      set_error("Scratch: too many allocations");
+      return FH_ERR_INTERNAL;
+    }
+    ptrs_[n_++] = p;
+    *out = static_cast<T*>(p);
+    return FH_OK;
+  }
+  void release() {
+    for (int i = n_ - 1; i >= 0; --i) (void)hipFreeAsync(ptrs_[i], stream_);
+    n_ = 0;
+  }
+
+ private:
+  static constexpr int kMax = 64;
+  hipStream_t stream_;
+  void* ptrs_[kMax] = {};
+  int n_ = 0;
+};
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Clears `count` elements with a kernel (no memset nodes, so callers may
+// capture the work into a graph).
+int fill_u32(uint32_t* p, uint32_t value, int64_t count, hipStream_t s);
+int fill_u64(uint64_t* p, uint64_t value, int64_t count, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Scan and radix sort (fh_scan_sort.hip)
+// ---------------------------------------------------------------------------
+// Exclusive prefix sum of `in` (n elements) into `out` (n elements); writes
+// the total to `total` (device pointer, may be null). In-place is allowed.
+int exclusive_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, uint64_t* total,
+                       hipStream_t s);
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total,
+                       hipStream_t s);
+
+// Stable LSD radix sort of 64-bit keys (optionally carrying 64-bit values) on
+// bits [begin_bit, end_bit). Results land in keys (and vals); tmp buffers of
+// the same sizes are caller-provided.
+int radix_sort_u64(uint64_t* keys, uint64_t* keys_tmp, uint64_t* vals, uint64_t* vals_tmp,
+                   int64_t n, int begin_bit, int end_bit, hipStream_t s);
+// Same for 32-bit keys with 64-bit values.
+int radix_sort_u32(uint32_t* keys, uint32_t* keys_tmp, uint64_t* vals, uint64_t* vals_tmp,
+                   int64_t n, int begin_bit, int end_bit, hipStream_t s);
+
+inline int bits_for(uint64_t max_value) {
+  int b = 0;
+  while (b < 64 && (max_value >> b) != 0) ++b;
+  return b;
+}
+
+}  // namespace fh
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+namespace fhd {
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const uint32_t lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+__device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
+
+}  // namespace fhd

@@ -1,0 +1,628 @@
+// Pair-list stages of the hot path: unique-value expansion, exact-equality
+// pair generation, level split, sorted-set membership and the pattern merge.
+//
+// All lists are uint64 "level keys" (pair id * 4 + level). The expansion and
+// exact-pair kernels are load-balanced over their outputs (one thread per
+// output element, which finds its input by binary search over the scanned
+// counts), so a value shared by thousands of rows does not serialise one
+// thread the way the reference's per-input loops do
+// (faster/comparison.py:236-241, faster/deduplication.py:295-332).
+#include <algorithm>
+#include <vector>
+
+#include "fh_common.h"
+
+namespace fh {
+namespace {
+
+constexpr int kThreads = 256;
+
+// Largest index i in [0, n) with offs[i] <= o (offs ascending, offs[0] = 0).
+template <typename T>
+__device__ __forceinline__ int64_t upper_index(const T* __restrict__ offs, int64_t n, uint64_t o) {
+  int64_t lo = 0, hi = n;  // invariant: offs[lo] <= o < offs[hi] (offs[n] = +inf)
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((uint64_t)offs[mid] <= o)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+inline unsigned grid_for(int64_t n, int64_t cap = 1 << 20) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kThreads), cap));
+}
+
+// --------------------------- linkage expansion -----------------------------
+__global__ void __launch_bounds__(kThreads) k_expand_link_count(const uint64_t* __restrict__ hits, int64_t n_hits,
+                                                                int64_t n_ub, const int64_t* __restrict__ starts_a,
+                                                                const int64_t* __restrict__ starts_b,
+                                                                uint64_t* __restrict__ cnt) {
+  for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < n_hits; h += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t id = hits[h] >> 2;
+    const uint64_t ua = id / (uint64_t)n_ub, ub = id % (uint64_t)n_ub;
+    cnt[h] = (uint64_t)(starts_a[ua + 1] - starts_a[ua]) * (uint64_t)(starts_b[ub + 1] - starts_b[ub]);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_expand_link(const uint64_t* __restrict__ hits, int64_t n_hits,
+                                                          int64_t n_ub, const int64_t* __restrict__ starts_a,
+                                                          const int64_t* __restrict__ rows_a,
+                                                          const int64_t* __restrict__ starts_b,
+                                                          const int64_t* __restrict__ rows_b, int64_t n_b,
+                                                          const uint64_t* __restrict__ offs, int64_t total,
+                                                          uint64_t* __restrict__ out) {
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t h = upper_index(offs, n_hits, (uint64_t)o);
+    const uint64_t key = hits[h];
+    const uint64_t id = key >> 2;
+    const uint64_t ua = id / (uint64_t)n_ub, ub = id % (uint64_t)n_ub;
+    const uint64_t k = (uint64_t)o - offs[h];
+    const uint64_t cb = (uint64_t)(starts_b[ub + 1] - starts_b[ub]);
+    const int64_t ra = rows_a[starts_a[ua] + (int64_t)(k / cb)];
+    const int64_t rb = rows_b[starts_b[ub] + (int64_t)(k % cb)];
+    out[o] = ((uint64_t)ra * (uint64_t)n_b + (uint64_t)rb) * 4ull + (key & 3ull);
+  }
+}
+
+// -------------------------- deduplication expansion ------------------------
+// Hit index h < n_hits: a value pair; h >= n_hits: value u = h - n_hits paired
+// with itself (the reference's forced diagonal) at diag_level.
+__device__ __forceinline__ void dedup_hit(const uint64_t* hits, int64_t n_hits, int64_t n_u, int diag_level,
+                                          int64_t h, uint64_t* ux, uint64_t* uy, uint64_t* lvl) {
+  if (h < n_hits) {
+    const uint64_t key = hits[h];
+    const uint64_t id = key >> 2;
+    *ux = id / (uint64_t)n_u;
+    *uy = id % (uint64_t)n_u;
+    *lvl = key & 3ull;
+  } else {
+    *ux = *uy = (uint64_t)(h - n_hits);
+    *lvl = (uint64_t)diag_level;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_expand_dedup_count(const uint64_t* __restrict__ hits, int64_t n_hits,
+                                                                 int64_t n_virtual, int64_t n_u, int diag_level,
+                                                                 const int64_t* __restrict__ starts,
+                                                                 uint64_t* __restrict__ cnt) {
+  for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < n_virtual; h += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t ux, uy, lvl;
+    dedup_hit(hits, n_hits, n_u, diag_level, h, &ux, &uy, &lvl);
+    const uint64_t cx = (uint64_t)(starts[ux + 1] - starts[ux]);
+    const uint64_t cy = (uint64_t)(starts[uy + 1] - starts[uy]);
+    cnt[h] = ux != uy ? cx * cy : cx * (cx - 1) / 2;
+  }
+}
+
+// k-th pair (i > j) of a group in row-major order of the strict lower
+// triangle: i = the largest with i(i-1)/2 <= k.
+__device__ __forceinline__ void tri_decode(uint64_t k, uint64_t* i, uint64_t* j) {
+  uint64_t r = (uint64_t)((1.0 + sqrt(1.0 + 8.0 * (double)k)) * 0.5);
+  while (r > 1 && r * (r - 1) / 2 > k) --r;
+  while ((r + 1) * r / 2 <= k) ++r;
+  *i = r;
+  *j = k - r * (r - 1) / 2;
+}
+
+__global__ void __launch_bounds__(kThreads) k_expand_dedup(const uint64_t* __restrict__ hits, int64_t n_hits,
+                                                           int64_t n_virtual, int64_t n_u, int diag_level,
+                                                           const int64_t* __restrict__ starts,
+                                                           const int64_t* __restrict__ rows, int64_t n,
+                                                           const uint64_t* __restrict__ offs, int64_t total,
+                                                           uint64_t* __restrict__ out) {
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t h = upper_index(offs, n_virtual, (uint64_t)o);
+    uint64_t ux, uy, lvl;
+    dedup_hit(hits, n_hits, n_u, diag_level, h, &ux, &uy, &lvl);
+    const uint64_t k = (uint64_t)o - offs[h];
+    uint64_t r1, r2;
+    if (ux != uy) {
+      const uint64_t cy = (uint64_t)(starts[uy + 1] - starts[uy]);
+      r1 = (uint64_t)rows[starts[ux] + (int64_t)(k / cy)];
+      r2 = (uint64_t)rows[starts[uy] + (int64_t)(k % cy)];
+    } else {
+      uint64_t i, j;
+      tri_decode(k, &i, &j);
+      r1 = (uint64_t)rows[starts[ux] + (int64_t)i];
+      r2 = (uint64_t)rows[starts[ux] + (int64_t)j];
+    }
+    const uint64_t hi = r1 > r2 ? r1 : r2, lo = r1 > r2 ? r2 : r1;
+    out[o] = (hi * (uint64_t)n + lo) * 4ull + lvl;
+  }
+}
+
+// ------------------------------ exact pairs --------------------------------
+__global__ void __launch_bounds__(kThreads) k_exact_link_count(const int64_t* __restrict__ group_a, int64_t n_a,
+                                                               const int64_t* __restrict__ starts_b,
+                                                               uint64_t* __restrict__ cnt) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_a; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = group_a[i];
+    cnt[i] = g >= 0 ? (uint64_t)(starts_b[g + 1] - starts_b[g]) : 0ull;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_exact_link(const int64_t* __restrict__ group_a, int64_t n_a,
+                                                         const int64_t* __restrict__ starts_b,
+                                                         const int64_t* __restrict__ rows_b, int64_t n_b,
+                                                         const uint64_t* __restrict__ offs, int64_t total,
+                                                         uint64_t* __restrict__ out) {
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = upper_index(offs, n_a, (uint64_t)o);
+    const int64_t g = group_a[i];
+    const int64_t j = rows_b[starts_b[g] + (o - (int64_t)offs[i])];
+    out[o] = ((uint64_t)i * (uint64_t)n_b + (uint64_t)j) * 4ull + 1ull;
+  }
+}
+
+// Position of every row inside its value group (rows ascending per group).
+__global__ void __launch_bounds__(kThreads) k_group_rank(const int64_t* __restrict__ group,
+                                                         const int64_t* __restrict__ starts,
+                                                         const int64_t* __restrict__ rows, int64_t n,
+                                                         uint64_t* __restrict__ rank) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = rows[k];
+    rank[r] = (uint64_t)(k - starts[group[r]]);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_exact_dedup(const int64_t* __restrict__ group, int64_t n,
+                                                          const int64_t* __restrict__ starts,
+                                                          const int64_t* __restrict__ rows,
+                                                          const uint64_t* __restrict__ offs, int64_t total,
+                                                          uint64_t* __restrict__ out) {
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = upper_index(offs, n, (uint64_t)o);
+    const int64_t j = rows[starts[group[i]] + (o - (int64_t)offs[i])];
+    out[o] = ((uint64_t)i * (uint64_t)n + (uint64_t)j) * 4ull + 1ull;
+  }
+}
+
+// ----------------------------- level split ---------------------------------
+__global__ void __launch_bounds__(kThreads) k_level_flags(const uint64_t* __restrict__ keys, int64_t n, int level,
+                                                          uint32_t* __restrict__ flags) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    flags[i] = (int)(keys[i] & 3ull) == level ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(kThreads) k_level_scatter(const uint64_t* __restrict__ keys, int64_t n, int level,
+                                                            const uint32_t* __restrict__ pos,
+                                                            int64_t* __restrict__ ids) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if ((int)(keys[i] & 3ull) == level) ids[pos[i]] = (int64_t)(keys[i] >> 2);
+}
+
+// ------------------------ sorted-set membership ----------------------------
+__global__ void __launch_bounds__(kThreads) k_member_mask(const int64_t* __restrict__ a1, int64_t n1,
+                                                          const int64_t* __restrict__ a2, int64_t n2, int inside,
+                                                          uint8_t* __restrict__ mask) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n1; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t x = a1[i];
+    int64_t lo = 0, hi = n2;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (a2[mid] < x)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    const bool found = lo < n2 && a2[lo] == x;
+    mask[i] = (uint8_t)(inside ? found : !found);
+  }
+}
+
+// ----------------------------- pattern merge -------------------------------
+// For each entry of the source list of variable `v`, finds the pair's level in
+// every other variable's list (0 when absent). An entry is the owner of its
+// pair when no earlier variable lists the pair; owners get their pattern code,
+// the others 0xFFFFFFFF.
+//
+// Each block handles a tile of consecutive source entries. For every other
+// list it locates the sub-range of keys that can match the tile's id range
+// with two binary searches; when the sub-range fits in LDS it is staged there
+// and searched locally, otherwise entries search the sub-range in place.
+constexpr int kMergeTile = 1024;
+constexpr int kMergeLds = 4096;
+constexpr int kMaxVars = 16;
+
+struct MergeArgs {
+  const uint64_t* lists[kMaxVars];
+  int64_t sizes[kMaxVars];
+  uint32_t stride[kMaxVars];
+  int n_vars;
+};
+
+__device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* a, int64_t lo, int64_t hi, uint64_t x) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, uint32_t* __restrict__ codes) {
+  __shared__ uint64_t seg[kMergeLds];
+  __shared__ int64_t range[2];
+  const uint64_t* src = m.lists[v];
+  const int64_t n = m.sizes[v];
+  const int64_t t0 = (int64_t)blockIdx.x * kMergeTile;
+  const int64_t t1 = min(t0 + (int64_t)kMergeTile, n);
+  constexpr int kPer = kMergeTile / kThreads;
+  uint64_t id[kPer];
+  uint32_t code[kPer];
+  bool owner[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int64_t e = t0 + k * kThreads + threadIdx.x;
+    const uint64_t key = e < t1 ? src[e] : 0ull;
+    id[k] = key >> 2;
+    code[k] = (uint32_t)(key & 3ull) * m.stride[v];
+    owner[k] = e < t1;
+  }
+  const uint64_t id_lo = src[t0] >> 2;
+  const uint64_t id_hi = src[t1 - 1] >> 2;
+  for (int w = 0; w < m.n_vars; ++w) {
+    if (w == v) continue;
+    const uint64_t* tl = m.lists[w];
+    if (threadIdx.x == 0) {
+      range[0] = lower_bound_u64(tl, 0, m.sizes[w], id_lo * 4ull);
+      range[1] = lower_bound_u64(tl, range[0], m.sizes[w], id_hi * 4ull + 4ull);
+    }
+    __syncthreads();
+    const int64_t b0 = range[0], b1 = range[1];
+    const int64_t len = b1 - b0;
+    const bool staged = len <= kMergeLds;
+    if (staged) {
+      for (int64_t i = threadIdx.x; i < len; i += kThreads) seg[i] = tl[b0 + i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      if (!owner[k] && code[k] == 0xFFFFFFFFu) continue;
+      const uint64_t x = id[k] * 4ull;
+      uint64_t hit;
+      bool found;
+      if (staged) {
+        const int64_t p = lower_bound_u64(seg, 0, len, x);
+        found = p < len && (seg[p] >> 2) == id[k];
+        hit = found ? seg[p] : 0ull;
+      } else {
+        const int64_t p = lower_bound_u64(tl, b0, b1, x);
+        found = p < b1 && (tl[p] >> 2) == id[k];
+        hit = found ? tl[p] : 0ull;
+      }
+      if (found) {
+        if (w < v) {
+          owner[k] = false;
+          code[k] = 0xFFFFFFFFu;
+        } else if (owner[k]) {
+          code[k] += (uint32_t)(hit & 3ull) * m.stride[w];
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int64_t e = t0 + k * kThreads + threadIdx.x;
+    if (e < t1) codes[e] = owner[k] ? code[k] : 0xFFFFFFFFu;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_owner_flags(const uint32_t* __restrict__ codes, int64_t n,
+                                                          uint32_t* __restrict__ flags) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    flags[i] = codes[i] != 0xFFFFFFFFu ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(kThreads) k_owner_scatter(const uint64_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ codes, int64_t n,
+                                                            const uint32_t* __restrict__ pos,
+                                                            uint32_t* __restrict__ out_codes,
+                                                            uint64_t* __restrict__ out_ids) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t c = codes[i];
+    if (c != 0xFFFFFFFFu) {
+      out_codes[pos[i]] = c;
+      out_ids[pos[i]] = keys[i] >> 2;
+    }
+  }
+}
+
+// offsets[c - code_lo] = first index of code c in the sorted codes (c in
+// [code_lo, code_hi]).
+__global__ void __launch_bounds__(kThreads) k_code_bounds(const uint32_t* __restrict__ codes, int64_t n,
+                                                          uint32_t code_lo, uint32_t code_hi,
+                                                          int64_t* __restrict__ offsets) {
+  const int64_t c = (int64_t)code_lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > (int64_t)code_hi) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)codes[mid] < c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  offsets[c - code_lo] = lo;
+}
+
+// ----------------------------- posteriors ----------------------------------
+__global__ void __launch_bounds__(kThreads) k_pair_posteriors(const int64_t* __restrict__ offs, int n_patterns,
+                                                              const double* __restrict__ ksi,
+                                                              double* __restrict__ post, int64_t n_pairs) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_pairs; i += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = n_patterns;  // offs[lo] <= i < offs[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (offs[mid] <= i)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    post[i] = ksi[lo];
+  }
+}
+
+// Scans `cnt` (n entries) into offsets and returns the total on the host.
+int scan_total(uint64_t* cnt, int64_t n, uint64_t* total_dev, uint64_t* total, hipStream_t s) {
+  FH_TRY(exclusive_scan_u64(cnt, cnt, n, total_dev, s));
+  FH_HIP(hipMemcpyAsync(total, total_dev, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  return FH_OK;
+}
+
+}  // namespace
+}  // namespace fh
+
+using namespace fh;
+
+extern "C" int fh_expand_link(const uint64_t* hits, int64_t n_hits, int64_t n_ub, const int64_t* starts_a,
+                              const int64_t* rows_a, const int64_t* starts_b, const int64_t* rows_b, int64_t n_b,
+                              uint64_t* out, int64_t capacity, int64_t* n_out, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_out != nullptr, "fh_expand_link: n_out is null");
+  FH_REQUIRE(n_hits >= 0 && n_ub > 0 && n_b >= 0, "fh_expand_link: bad sizes");
+  *n_out = 0;
+  if (n_hits == 0) return FH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Scratch scr(s);
+  uint64_t *cnt = nullptr, *tot = nullptr;
+  FH_TRY(scr.alloc(&cnt, (size_t)n_hits));
+  FH_TRY(scr.alloc(&tot, 1));
+  hipLaunchKernelGGL(k_expand_link_count, dim3(grid_for(n_hits)), dim3(kThreads), 0, s, hits, n_hits, n_ub, starts_a,
+                     starts_b, cnt);
+  FH_HIP(hipGetLastError());
+  uint64_t total = 0;
+  FH_TRY(scan_total(cnt, n_hits, tot, &total, s));
+  *n_out = (int64_t)total;
+  if ((int64_t)total > capacity || total == 0) return FH_OK;
+  hipLaunchKernelGGL(k_expand_link, dim3(grid_for((int64_t)total)), dim3(kThreads), 0, s, hits, n_hits, n_ub,
+                     starts_a, rows_a, starts_b, rows_b, n_b, cnt, (int64_t)total, out);
+  FH_HIP(hipGetLastError());
+  FH_HIP(hipStreamSynchronize(s));
+  return FH_OK;
+}
+
+extern "C" int fh_expand_dedup(const uint64_t* hits, int64_t n_hits, int64_t n_u, const int64_t* starts,
+                               const int64_t* rows, int64_t n, int diag_level, uint64_t* out, int64_t capacity,
+                               int64_t* n_out, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_out != nullptr, "fh_expand_dedup: n_out is null");
+  FH_REQUIRE(n_hits >= 0 && n_u >= 0 && n >= 0, "fh_expand_dedup: bad sizes");
+  *n_out = 0;
+  const int64_t n_virtual = n_hits + (diag_level > 0 ? n_u : 0);
+  if (n_virtual == 0) return FH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Scratch scr(s);
+  uint64_t *cnt = nullptr, *tot = nullptr;
+  FH_TRY(scr.alloc(&cnt, (size_t)n_virtual));
+  FH_TRY(scr.alloc(&tot, 1));
+  hipLaunchKernelGGL(k_expand_dedup_count, dim3(grid_for(n_virtual)), dim3(kThreads), 0, s, hits, n_hits, n_virtual,
+                     n_u, diag_level, starts, cnt);
+  FH_HIP(hipGetLastError());
+  uint64_t total = 0;
+  FH_TRY(scan_total(cnt, n_virtual, tot, &total, s));
+  *n_out = (int64_t)total;
+  if ((int64_t)total > capacity || total == 0) return FH_OK;
+  hipLaunchKernelGGL(k_expand_dedup, dim3(grid_for((int64_t)total)), dim3(kThreads), 0, s, hits, n_hits, n_virtual,
+                     n_u, diag_level, starts, rows, n, cnt, (int64_t)total, out);
+  FH_HIP(hipGetLastError());
+  FH_HIP(hipStreamSynchronize(s));
+  return FH_OK;
+}
+
+extern "C" int fh_exact_link(const int64_t* group_a, int64_t n_a, const int64_t* starts_b, const int64_t* rows_b,
+                             int64_t n_groups, int64_t n_b, uint64_t* out, int64_t capacity, int64_t* n_out,
+                             fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_out != nullptr, "fh_exact_link: n_out is null");
+  FH_REQUIRE(n_a >= 0 && n_groups >= 0 && n_b >= 0, "fh_exact_link: bad sizes");
+  (void)n_groups;
+  *n_out = 0;
+  if (n_a == 0) return FH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Scratch scr(s);
+  uint64_t *cnt = nullptr, *tot = nullptr;
+  FH_TRY(scr.alloc(&cnt, (size_t)n_a));
+  FH_TRY(scr.alloc(&tot, 1));
+  hipLaunchKernelGGL(k_exact_link_count, dim3(grid_for(n_a)), dim3(kThreads), 0, s, group_a, n_a, starts_b, cnt);
+  FH_HIP(hipGetLastError());
+  uint64_t total = 0;
+  FH_TRY(scan_total(cnt, n_a, tot, &total, s));
+  *n_out = (int64_t)total;
+  if ((int64_t)total > capacity || total == 0) return FH_OK;
+  hipLaunchKernelGGL(k_exact_link, dim3(grid_for((int64_t)total)), dim3(kThreads), 0, s, group_a, n_a, starts_b,
+                     rows_b, n_b, cnt, (int64_t)total, out);
+  FH_HIP(hipGetLastError());
+  FH_HIP(hipStreamSynchronize(s));
+  return FH_OK;
+}
+
+extern "C" int fh_exact_dedup(const int64_t* group, int64_t n, const int64_t* starts, const int64_t* rows,
+                              int64_t n_groups, uint64_t* out, int64_t capacity, int64_t* n_out,
+                              fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_out != nullptr, "fh_exact_dedup: n_out is null");
+  FH_REQUIRE(n >= 0 && n_groups >= 0, "fh_exact_dedup: bad sizes");
+  *n_out = 0;
+  if (n == 0) return FH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Scratch scr(s);
+  uint64_t *cnt = nullptr, *tot = nullptr;
+  FH_TRY(scr.alloc(&cnt, (size_t)n));
+  FH_TRY(scr.alloc(&tot, 1));
+  hipLaunchKernelGGL(k_group_rank, dim3(grid_for(n)), dim3(kThreads), 0, s, group, starts, rows, n, cnt);
+  FH_HIP(hipGetLastError());
+  uint64_t total = 0;
+  FH_TRY(scan_total(cnt, n, tot, &total, s));
+  *n_out = (int64_t)total;
+  if ((int64_t)total > capacity || total == 0) return FH_OK;
+  hipLaunchKernelGGL(k_exact_dedup, dim3(grid_for((int64_t)total)), dim3(kThreads), 0, s, group, n, starts, rows,
+                     cnt, (int64_t)total, out);
+  FH_HIP(hipGetLastError());
+  FH_HIP(hipStreamSynchronize(s));
+  return FH_OK;
+}
+
+extern "C" int fh_split_levels(const uint64_t* keys, int64_t n, int n_levels, int64_t* ids, int64_t* level_offsets,
+                               fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n >= 0 && n_levels >= 1 && n_levels <= 4, "fh_split_levels: bad arguments");
+  FH_REQUIRE(n < (int64_t)0xFFFFFFFF, "fh_split_levels: list too long");
+  FH_REQUIRE(level_offsets != nullptr, "fh_split_levels: level_offsets is null");
+  hipStream_t s = (hipStream_t)stream;
+  level_offsets[0] = 0;
+  level_offsets[1] = 0;
+  if (n == 0) {
+    for (int l = 1; l <= n_levels; ++l) level_offsets[l] = 0;
+    return FH_OK;
+  }
+  Scratch scr(s);
+  uint32_t *flags = nullptr, *tot = nullptr;
+  FH_TRY(scr.alloc(&flags, (size_t)n));
+  FH_TRY(scr.alloc(&tot, 1));
+  int64_t base = 0;
+  for (int l = 1; l < n_levels; ++l) {
+    hipLaunchKernelGGL(k_level_flags, dim3(grid_for(n)), dim3(kThreads), 0, s, keys, n, l, flags);
+    FH_HIP(hipGetLastError());
+    FH_TRY(exclusive_scan_u32(flags, flags, n, tot, s));
+    hipLaunchKernelGGL(k_level_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, keys, n, l, flags, ids + base);
+    FH_HIP(hipGetLastError());
+    uint32_t c = 0;
+    FH_HIP(hipMemcpyAsync(&c, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FH_HIP(hipStreamSynchronize(s));
+    base += c;
+    level_offsets[l + 1] = base;
+  }
+  return FH_OK;
+}
+
+extern "C" int fh_sorted_member_mask(const int64_t* arr1, int64_t n1, const int64_t* arr2, int64_t n2, int inside,
+                                     uint8_t* mask, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n1 >= 0 && n2 >= 0, "fh_sorted_member_mask: bad sizes");
+  if (n1 == 0) return FH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_member_mask, dim3(grid_for(n1)), dim3(kThreads), 0, s, arr1, n1, arr2, n2, inside, mask);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+extern "C" int fh_pattern_merge(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
+                                const int32_t* var_levels, int64_t* ids, int64_t capacity, int64_t* pattern_offsets,
+                                int64_t* n_union, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_vars >= 1 && n_vars <= kMaxVars, "fh_pattern_merge: 1..%d variables supported", kMaxVars);
+  FH_REQUIRE(pattern_offsets != nullptr && n_union != nullptr, "fh_pattern_merge: null host outputs");
+  hipStream_t s = (hipStream_t)stream;
+  MergeArgs m;
+  m.n_vars = n_vars;
+  uint64_t n_patterns = 1;
+  for (int v = 0; v < n_vars; ++v) {
+    FH_REQUIRE(var_levels[v] >= 2 && var_levels[v] <= 4, "fh_pattern_merge: levels must be 2..4");
+    FH_REQUIRE(var_sizes[v] >= 0 && var_sizes[v] < (int64_t)0xFFFFFFFF, "fh_pattern_merge: list too long");
+    n_patterns *= (uint64_t)var_levels[v];
+    FH_REQUIRE(n_patterns < (1ull << 31), "fh_pattern_merge: too many patterns");
+  }
+  for (int v = n_vars - 1, st = 1; v >= 0; --v) {
+    m.stride[v] = (uint32_t)st;
+    m.lists[v] = var_keys[v];
+    m.sizes[v] = var_sizes[v];
+    st *= var_levels[v];
+  }
+  for (uint64_t c = 0; c <= n_patterns; ++c) pattern_offsets[c] = 0;
+  Scratch scr(s);
+  int64_t nmax = 0;
+  for (int v = 0; v < n_vars; ++v) nmax = std::max(nmax, var_sizes[v]);
+  uint32_t *codes = nullptr, *flags = nullptr, *ocodes = nullptr, *ocodes_tmp = nullptr, *tot = nullptr;
+  uint64_t *oids = nullptr, *oids_tmp = nullptr;
+  int64_t* bounds = nullptr;
+  FH_TRY(scr.alloc(&codes, (size_t)std::max<int64_t>(1, nmax)));
+  FH_TRY(scr.alloc(&flags, (size_t)std::max<int64_t>(1, nmax)));
+  FH_TRY(scr.alloc(&ocodes, (size_t)std::max<int64_t>(1, nmax)));
+  FH_TRY(scr.alloc(&ocodes_tmp, (size_t)std::max<int64_t>(1, nmax)));
+  FH_TRY(scr.alloc(&oids_tmp, (size_t)std::max<int64_t>(1, nmax)));
+  FH_TRY(scr.alloc(&tot, 1));
+  FH_TRY(scr.alloc(&bounds, (size_t)n_patterns + 1));
+  const int code_bits = bits_for(n_patterns - 1);
+  int64_t base = 0;
+  // Codes owned by variable v lie in [stride_v, stride_{v-1}); the output is
+  // pattern-major, so variables are processed from the last to the first.
+  for (int v = n_vars - 1; v >= 0; --v) {
+    const int64_t n = var_sizes[v];
+    const uint32_t code_lo = m.stride[v];
+    const uint32_t code_hi = (v == 0 ? (uint32_t)n_patterns : m.stride[v - 1]) - 1;
+    if (n == 0) {
+      for (uint32_t c = code_lo; c <= code_hi; ++c) pattern_offsets[c] = base;
+      continue;
+    }
+    hipLaunchKernelGGL(k_pattern_codes, dim3((unsigned)ceil_div(n, kMergeTile)), dim3(kThreads), 0, s, m, v, codes);
+    FH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_owner_flags, dim3(grid_for(n)), dim3(kThreads), 0, s, codes, n, flags);
+    FH_HIP(hipGetLastError());
+    FH_TRY(exclusive_scan_u32(flags, flags, n, tot, s));
+    uint32_t owners = 0;
+    FH_HIP(hipMemcpyAsync(&owners, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    FH_HIP(hipStreamSynchronize(s));
+    if (base + (int64_t)owners <= capacity && owners > 0) {
+      oids = reinterpret_cast<uint64_t*>(ids + base);
+      hipLaunchKernelGGL(k_owner_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, m.lists[v], codes, n, flags,
+                         ocodes, oids);
+      FH_HIP(hipGetLastError());
+      FH_TRY(radix_sort_u32(ocodes, ocodes_tmp, oids, oids_tmp, owners, 0, code_bits, s));
+      const uint32_t ncodes = code_hi - code_lo + 1;
+      hipLaunchKernelGGL(k_code_bounds, dim3((unsigned)ceil_div(ncodes, kThreads)), dim3(kThreads), 0, s, ocodes,
+                         (int64_t)owners, code_lo, code_hi, bounds);
+      FH_HIP(hipGetLastError());
+      std::vector<int64_t> hb(ncodes);
+      FH_HIP(hipMemcpyAsync(hb.data(), bounds, ncodes * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+      FH_HIP(hipStreamSynchronize(s));
+      for (uint32_t c = 0; c < ncodes; ++c) pattern_offsets[code_lo + c] = base + hb[c];
+    } else {
+      for (uint32_t c = code_lo; c <= code_hi; ++c) pattern_offsets[c] = base;
+    }
+    base += owners;
+  }
+  pattern_offsets[0] = 0;
+  pattern_offsets[n_patterns] = base;
+  *n_union = base;
+  return FH_OK;
+}
+
+extern "C" int fh_pair_posteriors(const int64_t* pattern_offsets, int n_patterns, const double* ksi, double* post,
+                                  int64_t n_pairs, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_patterns >= 1 && n_pairs >= 0, "fh_pair_posteriors: bad sizes");
+  if (n_pairs == 0) return FH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pair_posteriors, dim3(grid_for(n_pairs)), dim3(kThreads), 0, s, pattern_offsets, n_patterns,
+                     ksi, post, n_pairs);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}

@@ -1,0 +1,113 @@
+"""ctypes binding of libfaster_hip.so (the C ABI declared in include/faster_hip.h).
+
+The library is built in-tree by ``fast-er_amd/build_hip.py`` and loaded from
+this directory. There is no fallback: when the library or a ROCm GPU is
+missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfaster_hip.so")
+
+FH_MODE_LINK = 0
+FH_MODE_DEDUP = 1
+
+_lock = threading.Lock()
+_lib = None
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+_f32 = ctypes.c_float
+_f64 = ctypes.c_double
+
+
+class JwStats(ctypes.Structure):
+    """Mirror of fh_jw_stats."""
+
+    _fields_ = [
+        ("pairs_logical", ctypes.c_int64),
+        ("pairs_kernel", ctypes.c_int64),
+        ("pairs_pruned", ctypes.c_int64),
+        ("pairs_generic", ctypes.c_int64),
+        ("alg_lane_ops", ctypes.c_double),
+        ("alphabet", ctypes.c_int32),
+        ("launches", ctypes.c_int32),
+        ("kernel_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_SIGS = {
+    "fh_last_error": (ctypes.c_char_p, []),
+    "fh_version": (_i32, []),
+    "fh_jw_levels": (_i32, [_vp, _vp, _i64, _vp, _vp, _i64, _i32, _f32, _f32, _f32, _vp, _i64,
+                            ctypes.POINTER(_i64), ctypes.POINTER(JwStats), _vp]),
+    "fh_expand_link": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
+                              ctypes.POINTER(_i64), _vp]),
+    "fh_expand_dedup": (_i32, [_vp, _i64, _i64, _vp, _vp, _i64, _i32, _vp, _i64,
+                               ctypes.POINTER(_i64), _vp]),
+    "fh_exact_link": (_i32, [_vp, _i64, _vp, _vp, _i64, _i64, _vp, _i64, ctypes.POINTER(_i64), _vp]),
+    "fh_exact_dedup": (_i32, [_vp, _i64, _vp, _vp, _i64, _vp, _i64, ctypes.POINTER(_i64), _vp]),
+    "fh_sort_u64": (_i32, [_vp, _i64, _i32, _i32, _vp]),
+    "fh_split_levels": (_i32, [_vp, _i64, _i32, _vp, ctypes.POINTER(_i64), _vp]),
+    "fh_sorted_member_mask": (_i32, [_vp, _i64, _vp, _i64, _i32, _vp, _vp]),
+    "fh_pattern_merge": (_i32, [_i32, ctypes.POINTER(_vp), ctypes.POINTER(_i64),
+                                ctypes.POINTER(ctypes.c_int32), _vp, _i64, ctypes.POINTER(_i64),
+                                ctypes.POINTER(_i64), _vp]),
+    "fh_em_fit": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _f64, _i32, _vp,
+                         ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp]),
+    "fh_match_probability": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _vp]),
+    "fh_pair_posteriors": (_i32, [_vp, _i32, _vp, _vp, _i64, _vp]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+def load(path: str = LIB_PATH, require_gpu: bool = True):
+    """Load the library (once). With ``require_gpu`` the call fails unless a
+    ROCm device is visible to torch."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(path):
+                raise RuntimeError(
+                    f"libfaster_hip.so not found at {path}; build it with "
+                    "`python fast-er_amd/build_hip.py`")
+            L = ctypes.CDLL(path)
+            for name, (res, args) in _SIGS.items():
+                fn = getattr(L, name)
+                fn.restype = res
+                fn.argtypes = args
+            _lib = L
+    if require_gpu and not torch.cuda.is_available():
+        raise RuntimeError("faster (MI355X build) needs a ROCm GPU; none is visible")
+    return _lib
+
+
+def check(status: int, what: str) -> None:
+    if status != 0:
+        msg = _lib.fh_last_error().decode(errors="replace") if _lib is not None else ""
+        raise RuntimeError(f"{what} failed (status {status}): {msg}")
+
+
+def stream():
+    return _vp(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t: torch.Tensor | None):
+    if t is None:
+        return _vp(0)
+    return _vp(t.data_ptr())
+
+
+def device() -> torch.device:
+    return torch.device("cuda", torch.cuda.current_device())

@@ -1,0 +1,225 @@
+"""Thin torch-tensor wrappers over the C ABI (include/faster_hip.h).
+
+Every function takes and returns device tensors on the current ROCm device
+and runs on torch's current stream. Pair-list keys are int64 tensors holding
+the ABI's uint64 level keys (id * 4 + level, always below 2**63).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, ptr, stream
+
+
+def _i64(x) -> ctypes.c_int64:
+    return ctypes.c_int64(int(x))
+
+
+def _empty_i64(n: int) -> torch.Tensor:
+    return torch.empty(max(int(n), 1), dtype=torch.int64, device=_lib.device())
+
+
+def _retry(run, first_capacity: int):
+    """Runs ``run(capacity) -> (required, result)`` and, when the arena was too
+    small, once more with the exact size."""
+    cap = int(first_capacity)
+    need, res = run(cap)
+    if need > cap:
+        need2, res = run(int(need))
+        if need2 != need:
+            raise RuntimeError(f"output size changed between runs ({need} vs {need2})")
+    return need, res
+
+
+def bits_for(max_value: int) -> int:
+    return max(1, int(max_value).bit_length())
+
+
+def jw_levels(chars_a, off_a, n_a, chars_b, off_b, n_b, mode, p, lower_thr, upper_thr,
+              capacity=None, want_stats=False):
+    """All-pairs JW levels (fh_jw_levels). Returns (hit keys, stats dict or None)."""
+    L = _lib.load()
+    if capacity is None:
+        capacity = max(1 << 20, 4 * (int(n_a) + int(n_b)))
+    stats = _lib.JwStats()
+
+    def run(cap):
+        hits = _empty_i64(cap)
+        n = ctypes.c_int64(0)
+        check(L.fh_jw_levels(ptr(chars_a), ptr(off_a), _i64(n_a), ptr(chars_b), ptr(off_b), _i64(n_b),
+                             int(mode), ctypes.c_float(p), ctypes.c_float(lower_thr),
+                             ctypes.c_float(upper_thr), ptr(hits), _i64(cap), ctypes.byref(n),
+                             ctypes.byref(stats) if want_stats else None, stream()), "fh_jw_levels")
+        return n.value, hits
+
+    need, hits = _retry(run, capacity)
+    return hits[:need], (stats.as_dict() if want_stats else None)
+
+
+def expand_link(hits, n_ub, starts_a, rows_a, starts_b, rows_b, n_b):
+    L = _lib.load()
+    n_hits = hits.numel()
+    if n_hits == 0:
+        return _empty_i64(0)[:0]
+
+    def run(cap):
+        out = _empty_i64(cap)
+        n = ctypes.c_int64(0)
+        check(L.fh_expand_link(ptr(hits), _i64(n_hits), _i64(n_ub), ptr(starts_a), ptr(rows_a),
+                               ptr(starts_b), ptr(rows_b), _i64(n_b), ptr(out), _i64(cap),
+                               ctypes.byref(n), stream()), "fh_expand_link")
+        return n.value, out
+
+    need, out = _retry(run, max(2 * n_hits, 1024))
+    return out[:need]
+
+
+def expand_dedup(hits, n_u, starts, rows, n, diag_level):
+    L = _lib.load()
+    n_hits = hits.numel()
+
+    def run(cap):
+        out = _empty_i64(cap)
+        m = ctypes.c_int64(0)
+        check(L.fh_expand_dedup(ptr(hits) if n_hits else ptr(None), _i64(n_hits), _i64(n_u), ptr(starts),
+                                ptr(rows), _i64(n), int(diag_level), ptr(out), _i64(cap),
+                                ctypes.byref(m), stream()), "fh_expand_dedup")
+        return m.value, out
+
+    need, out = _retry(run, max(2 * n_hits, 1024))
+    return out[:need]
+
+
+def exact_link(group_a, starts_b, rows_b, n_groups, n_b, capacity=None):
+    L = _lib.load()
+    n_a = group_a.numel()
+
+    def run(cap):
+        out = _empty_i64(cap)
+        m = ctypes.c_int64(0)
+        check(L.fh_exact_link(ptr(group_a), _i64(n_a), ptr(starts_b), ptr(rows_b), _i64(n_groups),
+                              _i64(n_b), ptr(out), _i64(cap), ctypes.byref(m), stream()),
+              "fh_exact_link")
+        return m.value, out
+
+    need, out = _retry(run, capacity if capacity is not None else 0)
+    return out[:need]
+
+
+def exact_dedup(group, starts, rows, n_groups, capacity=None):
+    L = _lib.load()
+    n = group.numel()
+
+    def run(cap):
+        out = _empty_i64(cap)
+        m = ctypes.c_int64(0)
+        check(L.fh_exact_dedup(ptr(group), _i64(n), ptr(starts), ptr(rows), _i64(n_groups), ptr(out),
+                               _i64(cap), ctypes.byref(m), stream()), "fh_exact_dedup")
+        return m.value, out
+
+    need, out = _retry(run, capacity if capacity is not None else 0)
+    return out[:need]
+
+
+def sort_keys(keys: torch.Tensor, max_key: int) -> torch.Tensor:
+    """In-place ascending radix sort of level keys below ``max_key``."""
+    L = _lib.load()
+    n = keys.numel()
+    if n > 1:
+        check(L.fh_sort_u64(ptr(keys), _i64(n), 0, bits_for(max_key), stream()), "fh_sort_u64")
+    return keys
+
+
+def split_levels(keys: torch.Tensor, n_levels: int):
+    """Sorted level keys -> (ids, host offsets [n_levels + 1])."""
+    L = _lib.load()
+    n = keys.numel()
+    ids = _empty_i64(n)
+    offs = (ctypes.c_int64 * (n_levels + 1))()
+    check(L.fh_split_levels(ptr(keys) if n else ptr(None), _i64(n), int(n_levels), ptr(ids), offs,
+                            stream()), "fh_split_levels")
+    return ids[:n], [offs[i] for i in range(n_levels + 1)]
+
+
+def member_mask(arr1: torch.Tensor, arr2: torch.Tensor, inside: bool) -> torch.Tensor:
+    L = _lib.load()
+    n1, n2 = arr1.numel(), arr2.numel()
+    mask = torch.empty(max(n1, 1), dtype=torch.uint8, device=_lib.device())
+    a1 = arr1.contiguous().to(torch.int64)
+    a2 = arr2.contiguous().to(torch.int64)
+    check(L.fh_sorted_member_mask(ptr(a1), _i64(n1), ptr(a2) if n2 else ptr(None), _i64(n2),
+                                  int(bool(inside)), ptr(mask), stream()), "fh_sorted_member_mask")
+    return mask[:n1].bool()
+
+
+def pattern_merge(var_keys, var_levels):
+    """Per-variable sorted level keys -> (ids, host pattern offsets, n_union)."""
+    L = _lib.load()
+    nv = len(var_keys)
+    keys = [k.contiguous() for k in var_keys]
+    ptrs = (ctypes.c_void_p * nv)(*[k.data_ptr() if k.numel() else 0 for k in keys])
+    sizes = (ctypes.c_int64 * nv)(*[k.numel() for k in keys])
+    levels = (ctypes.c_int32 * nv)(*[int(x) for x in var_levels])
+    n_patterns = int(np.prod([int(x) for x in var_levels]))
+    cap = max(1, sum(k.numel() for k in keys))
+    ids = _empty_i64(cap)
+    offs = (ctypes.c_int64 * (n_patterns + 1))()
+    n_union = ctypes.c_int64(0)
+    check(L.fh_pattern_merge(nv, ptrs, sizes, levels, ptr(ids), _i64(cap), offs, ctypes.byref(n_union),
+                             stream()), "fh_pattern_merge")
+    if n_union.value > cap:
+        raise RuntimeError("pattern merge: union larger than the sum of its inputs")
+    return ids[:n_union.value], [offs[i] for i in range(n_patterns + 1)], n_union.value
+
+
+def _levels_arr(levels):
+    return (ctypes.c_int32 * len(levels))(*[int(x) for x in levels])
+
+
+def em_fit(levels, counts: torch.Tensor, params0: torch.Tensor, tol: float, max_iter: int):
+    """FP64 EM on the device. Returns (params, ksi, iterations, converged)."""
+    L = _lib.load()
+    n_patterns = int(np.prod([int(x) for x in levels]))
+    params = params0.clone().to(device=_lib.device(), dtype=torch.float64).contiguous()
+    ksi = torch.empty(n_patterns, dtype=torch.float64, device=_lib.device())
+    it = ctypes.c_int(0)
+    conv = ctypes.c_int(0)
+    c = counts.to(device=_lib.device(), dtype=torch.int64).contiguous()
+    check(L.fh_em_fit(len(levels), _levels_arr(levels), ptr(c), ptr(params), ctypes.c_double(tol),
+                      int(max_iter), ptr(ksi), ctypes.byref(it), ctypes.byref(conv), stream()),
+          "fh_em_fit")
+    return params, ksi, it.value, bool(conv.value)
+
+
+def match_probability(levels, params: torch.Tensor) -> torch.Tensor:
+    L = _lib.load()
+    n_patterns = int(np.prod([int(x) for x in levels]))
+    p = params.to(device=_lib.device(), dtype=torch.float64).contiguous()
+    ksi = torch.empty(n_patterns, dtype=torch.float64, device=_lib.device())
+    check(L.fh_match_probability(len(levels), _levels_arr(levels), ptr(p), ptr(ksi), stream()),
+          "fh_match_probability")
+    return ksi
+
+
+def pair_posteriors(pattern_offsets, ksi: torch.Tensor, n_pairs: int) -> torch.Tensor:
+    L = _lib.load()
+    offs = torch.tensor(pattern_offsets, dtype=torch.int64, device=_lib.device())
+    k = ksi.to(device=_lib.device(), dtype=torch.float64).contiguous()
+    post = torch.empty(max(int(n_pairs), 1), dtype=torch.float64, device=_lib.device())
+    check(L.fh_pair_posteriors(ptr(offs), len(pattern_offsets) - 1, ptr(k), ptr(post), _i64(n_pairs),
+                               stream()), "fh_pair_posteriors")
+    return post[:n_pairs]
+
+
+def max_pair_key(n_ids: int) -> int:
+    """Largest level key for ids below n_ids."""
+    return max(0, int(n_ids) - 1) * 4 + 3
+
+
+def ceil_log2(x: int) -> int:
+    return int(math.ceil(math.log2(max(2, x))))

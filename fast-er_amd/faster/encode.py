@@ -1,0 +1,164 @@
+"""Host-side column encoding: unique values, packed bytes and row groups.
+
+The reference takes unique values with ``np.unique`` on the host before any
+device work (faster/comparison.py:377,395, faster/deduplication.py:420); this
+module does the same once per column and produces the flat arrays the device
+stages consume:
+
+* packed UTF-8 bytes of the unique values with int64 offsets (the layout of
+  faster/comparison.py:283-302);
+* for every unique value, its rows in ascending order (CSR ``starts`` /
+  ``rows``), the role of the stable ``cp.argsort`` of the inverse at
+  faster/comparison.py:380-391.
+
+Unique values come out in ``np.unique`` order, i.e. lexicographic by code
+point, which equals byte order of their UTF-8 encodings.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+@dataclass
+class StringGroups:
+    n_rows: int
+    chars: np.ndarray    # uint8, packed unique values
+    offsets: np.ndarray  # int64[n_unique + 1]
+    starts: np.ndarray   # int64[n_unique + 1]
+    rows: np.ndarray     # int64[n_rows], rows of each unique value, ascending
+
+    @property
+    def n_unique(self) -> int:
+        return len(self.offsets) - 1
+
+    def to_device(self) -> "DeviceStringGroups":
+        dev = _lib.device()
+        chars = self.chars if self.chars.size else np.zeros(1, np.uint8)
+        return DeviceStringGroups(
+            n_rows=self.n_rows, n_unique=self.n_unique,
+            chars=torch.from_numpy(chars).to(dev),
+            offsets=torch.from_numpy(self.offsets).to(dev),
+            starts=torch.from_numpy(self.starts).to(dev),
+            rows=torch.from_numpy(self.rows).to(dev))
+
+
+@dataclass
+class DeviceStringGroups:
+    n_rows: int
+    n_unique: int
+    chars: torch.Tensor
+    offsets: torch.Tensor
+    starts: torch.Tensor
+    rows: torch.Tensor
+
+
+def _to_str_array(values) -> np.ndarray:
+    arr = np.asarray(values, dtype=object)
+    return arr.reshape(-1)
+
+
+def pack_strings(values) -> tuple[np.ndarray, np.ndarray]:
+    """Packs strings (UTF-8) into bytes + int64 offsets, no uniquing."""
+    arr = _to_str_array(values)
+    bs = [s.encode("utf-8") if isinstance(s, str) else bytes(s) for s in arr]
+    lens = np.fromiter((len(b) for b in bs), dtype=np.int64, count=len(bs))
+    offsets = np.zeros(len(bs) + 1, dtype=np.int64)
+    np.cumsum(lens, out=offsets[1:])
+    chars = np.frombuffer(b"".join(bs), dtype=np.uint8).copy()
+    return chars, offsets
+
+
+def group_strings(values) -> StringGroups:
+    """Unique values of a string column with their rows (ascending)."""
+    arr = _to_str_array(values)
+    n = len(arr)
+    if n == 0:
+        z = np.zeros(1, dtype=np.int64)
+        return StringGroups(0, np.zeros(0, np.uint8), z, z.copy(), np.zeros(0, np.int64))
+    as_str = [s if isinstance(s, str) else str(s) for s in arr]
+    has_nul = any("\x00" in s for s in as_str)
+    if not has_nul:
+        enc = np.array([s.encode("utf-8") for s in as_str], dtype=object)
+        fixed = enc.astype(bytes)
+        uniq, inv, cnt = np.unique(fixed, return_inverse=True, return_counts=True)
+        inv = np.asarray(inv).reshape(-1)
+        width = uniq.dtype.itemsize
+        ulens = np.char.str_len(uniq).astype(np.int64)
+        if width == 0 or len(uniq) == 0:
+            chars = np.zeros(0, np.uint8)
+        else:
+            mat = np.frombuffer(uniq.tobytes(), dtype=np.uint8).reshape(len(uniq), width)
+            chars = mat[np.arange(width)[None, :] < ulens[:, None]].copy()
+    else:
+        uniq_obj, inv, cnt = np.unique(np.array(as_str, dtype=object), return_inverse=True,
+                                       return_counts=True)
+        inv = np.asarray(inv).reshape(-1)
+        bs = [u.encode("utf-8") for u in uniq_obj]
+        ulens = np.fromiter((len(b) for b in bs), dtype=np.int64, count=len(bs))
+        chars = np.frombuffer(b"".join(bs), dtype=np.uint8).copy()
+    offsets = np.zeros(len(ulens) + 1, dtype=np.int64)
+    np.cumsum(ulens, out=offsets[1:])
+    rows = np.argsort(inv, kind="stable").astype(np.int64)
+    starts = np.zeros(len(cnt) + 1, dtype=np.int64)
+    np.cumsum(cnt, out=starts[1:])
+    return StringGroups(n, chars, offsets, starts, rows)
+
+
+@dataclass
+class ExactLinkGroups:
+    """Exact-match grouping of two columns over their joint values."""
+    group_a: np.ndarray   # int64[n_A]: joint value id of each A row
+    starts_b: np.ndarray  # int64[n_groups + 1]
+    rows_b: np.ndarray    # int64[n_B]: B rows of each joint value, ascending
+    n_groups: int
+
+    def to_device(self):
+        dev = _lib.device()
+        return (torch.from_numpy(self.group_a).to(dev), torch.from_numpy(self.starts_b).to(dev),
+                torch.from_numpy(self.rows_b).to(dev))
+
+
+def group_exact_link(values_a, values_b) -> ExactLinkGroups:
+    """Joint value ids of both columns (np.unique over their concatenation, as
+    faster/comparison.py:548 does over the two unique sets)."""
+    a = np.asarray(values_a).reshape(-1)
+    b = np.asarray(values_b).reshape(-1)
+    both = np.concatenate((a, b))
+    _, inv = np.unique(both, return_inverse=True)
+    inv = np.asarray(inv).reshape(-1).astype(np.int64)
+    inv_a, inv_b = inv[:len(a)], inv[len(a):]
+    n_groups = int(inv.max()) + 1 if len(inv) else 0
+    cnt_b = np.bincount(inv_b, minlength=n_groups).astype(np.int64)
+    starts_b = np.zeros(n_groups + 1, dtype=np.int64)
+    np.cumsum(cnt_b, out=starts_b[1:])
+    rows_b = np.argsort(inv_b, kind="stable").astype(np.int64)
+    return ExactLinkGroups(inv_a.copy(), starts_b, rows_b, n_groups)
+
+
+@dataclass
+class ExactDedupGroups:
+    group: np.ndarray   # int64[n]
+    starts: np.ndarray  # int64[n_groups + 1]
+    rows: np.ndarray    # int64[n]
+    n_groups: int
+
+    def to_device(self):
+        dev = _lib.device()
+        return (torch.from_numpy(self.group).to(dev), torch.from_numpy(self.starts).to(dev),
+                torch.from_numpy(self.rows).to(dev))
+
+
+def group_exact_dedup(values) -> ExactDedupGroups:
+    """Value ids of one column (np.unique, faster/deduplication.py:645)."""
+    v = np.asarray(values).reshape(-1)
+    _, inv, cnt = np.unique(v, return_inverse=True, return_counts=True)
+    inv = np.asarray(inv).reshape(-1).astype(np.int64)
+    starts = np.zeros(len(cnt) + 1, dtype=np.int64)
+    np.cumsum(cnt, out=starts[1:])
+    rows = np.argsort(inv, kind="stable").astype(np.int64)
+    return ExactDedupGroups(inv, starts, rows, len(cnt))

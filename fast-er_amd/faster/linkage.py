@@ -1,0 +1,51 @@
+"""Linkage: the ``Linkage`` class (drop-in for faster/linkage.py, fast-er 0.3.1)."""
+from __future__ import annotations
+
+import numpy as np
+import pandas as pd
+import torch
+
+from . import _lib
+
+
+def _numel(x) -> int:
+    return int(x.numel()) if isinstance(x, torch.Tensor) else int(np.asarray(x).size)
+
+
+class Linkage():
+    """Links records of two DataFrames whose pattern's match probability
+    reaches a threshold.
+
+    :param df_A: First DataFrame.
+    :param df_B: Second DataFrame.
+    :param Indices: Per-pattern pair lists (``Comparison.Indices``).
+    :param Ksi: Match probability of every pattern (``Estimation.Ksi``).
+    """
+
+    def __init__(self, df_A: pd.DataFrame, df_B: pd.DataFrame, Indices, Ksi: np.array):
+        self.df_A = df_A
+        self.df_B = df_B
+        self.Indices = Indices
+        self.Ksi = Ksi
+
+    def transform(self, Threshold=0.85):
+        """DataFrame of all pairs whose match probability is at least
+        ``Threshold`` (faster/linkage.py:26-72). As in the reference, pattern
+        ``k`` reads ``Indices[k - 1]``, so pattern 0 maps to the last list."""
+        df_A = self.df_A.add_suffix("_A")
+        df_B = self.df_B.add_suffix("_B")
+        df_A["Index_A"] = range(len(df_A))
+        df_B["Index_B"] = range(len(df_B))
+        above = np.ravel(np.argwhere(np.asarray(self.Ksi) >= Threshold))
+        if np.sum([_numel(self.Indices[i - 1]) for i in above]) == 0:
+            raise Exception("No pair of observations has a conditional match probability exceeding the threshold.")
+        dev = _lib.device()
+        parts = [x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x, dtype=np.int64))
+                 for x in (self.Indices[i - 1] for i in above)]
+        ids = torch.cat([x.to(dev, torch.int64) for x in parts])
+        n_b = len(df_B)
+        a = (ids // n_b).cpu().numpy()
+        b = (ids % n_b).cpu().numpy()
+        df_A = df_A.iloc[a, :]
+        df_A["Index_B"] = b
+        return df_A.merge(df_B, on="Index_B")

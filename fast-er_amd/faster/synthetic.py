@@ -1,0 +1,113 @@
+"""Seeded synthetic record tables for tests and benchmarks.
+
+Linkage tables: three string variables (first_name, last_name, street) of
+lowercase ASCII with lengths ``clip(round(lognormal(ln 7, 0.4)), 2, 20)`` and
+an int32 house number uniform in [1, 9999]. A fraction of B rows (30% by
+default) copy distinct A rows with 0..max_edits uniform edits per string
+(substitution, insertion, deletion or adjacent transposition, uniformly) and
+the same house number; the rest are independent. Deduplication tables add
+geometric clusters of perturbed copies and an int16 birth year.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pandas as pd
+
+_ALPHA = np.frombuffer(b"abcdefghijklmnopqrstuvwxyz", dtype=np.uint8)
+STRING_VARS = ("first_name", "last_name", "street")
+
+
+def random_strings(rng, n, mean_len=7.0, sigma=0.4, lo=2, hi=20):
+    """``n`` random lowercase strings (list of str)."""
+    lens = np.clip(np.round(rng.lognormal(np.log(mean_len), sigma, n)), lo, hi).astype(np.int64)
+    buf = _ALPHA[rng.integers(0, 26, int(lens.sum()))].tobytes().decode("ascii")
+    offs = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(lens, out=offs[1:])
+    return [buf[offs[i]:offs[i + 1]] for i in range(n)]
+
+
+def _edit(s: str, kind: int, pos_u: float, ch: int) -> str:
+    c = chr(97 + ch)
+    n = len(s)
+    if kind == 0 and n > 0:  # substitution
+        i = int(pos_u * n)
+        return s[:i] + c + s[i + 1:]
+    if kind == 1:  # insertion
+        i = int(pos_u * (n + 1))
+        return s[:i] + c + s[i:]
+    if kind == 2 and n > 1:  # deletion
+        i = int(pos_u * n)
+        return s[:i] + s[i + 1:]
+    if kind == 3 and n > 1:  # adjacent transposition
+        i = int(pos_u * (n - 1))
+        return s[:i] + s[i + 1] + s[i] + s[i + 2:]
+    return s
+
+
+def perturb_all(rng, strings, min_edits, max_edits):
+    """Applies a uniform number of edits in [min_edits, max_edits] to every string."""
+    n = len(strings)
+    k = rng.integers(min_edits, max_edits + 1, n)
+    tot = int(k.sum())
+    kinds = rng.integers(0, 4, tot)
+    pos = rng.random(tot)
+    chars = rng.integers(0, 26, tot)
+    out = []
+    j = 0
+    for s, ke in zip(strings, k.tolist()):
+        for _ in range(ke):
+            s = _edit(s, int(kinds[j]), float(pos[j]), int(chars[j]))
+            j += 1
+        out.append(s)
+    return out
+
+
+def linkage_tables(n_a, n_b, seed=0, dup_frac=0.3, max_edits=2):
+    """(df_A, df_B) for linkage. Returns also the A row of every copied B row
+    (-1 for independent rows)."""
+    rng = np.random.default_rng(seed)
+    A = {v: random_strings(rng, n_a) for v in STRING_VARS}
+    A["house_number"] = rng.integers(1, 10000, n_a).astype(np.int32)
+    n_dup = min(int(round(dup_frac * n_b)), n_a)
+    src = rng.choice(n_a, n_dup, replace=False)
+    dst = rng.choice(n_b, n_dup, replace=False)
+    B = {v: random_strings(rng, n_b) for v in STRING_VARS}
+    B["house_number"] = rng.integers(1, 10000, n_b).astype(np.int32)
+    truth = np.full(n_b, -1, dtype=np.int64)
+    truth[dst] = src
+    for v in STRING_VARS:
+        copies = perturb_all(rng, [A[v][i] for i in src.tolist()], 0, max_edits)
+        col = B[v]
+        for d, s in zip(dst.tolist(), copies):
+            col[d] = s
+    B["house_number"][dst] = A["house_number"][src]
+    return pd.DataFrame(A), pd.DataFrame(B), truth
+
+
+def dedup_table(n, seed=0, dup_frac=0.2, min_edits=1, max_edits=3, p_geom=0.5):
+    """One table for deduplication: ``dup_frac`` of the rows are perturbed
+    copies arranged in clusters of 2..5 rows (sizes geometric, truncated)."""
+    rng = np.random.default_rng(seed)
+    cols = {v: random_strings(rng, n) for v in STRING_VARS}
+    cols["birth_year"] = rng.integers(1930, 2006, n).astype(np.int16)
+    n_dup = int(round(dup_frac * n))
+    cluster = np.full(n, -1, dtype=np.int64)
+    perm = rng.permutation(n)
+    i = 0
+    cid = 0
+    while i < n_dup:
+        size = int(min(5, 1 + rng.geometric(p_geom)))
+        size = max(2, min(size, n_dup - i + 1, n - i))
+        if size < 2:
+            break
+        rows = perm[i:i + size]
+        root = rows[0]
+        for v in STRING_VARS:
+            copies = perturb_all(rng, [cols[v][root]] * (size - 1), min_edits, max_edits)
+            for r, s in zip(rows[1:].tolist(), copies):
+                cols[v][r] = s
+        cols["birth_year"][rows[1:]] = cols["birth_year"][root]
+        cluster[rows] = cid
+        cid += 1
+        i += size
+    return pd.DataFrame(cols), cluster

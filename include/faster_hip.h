@@ -1,0 +1,227 @@
+/*
+ * libfaster_hip -- C ABI of the MI355X (gfx950) linkage hot path.
+ *
+ * Conventions
+ *   - Every entry point returns an int status (FH_OK == 0). On failure a
+ *     thread-local message is available from fh_last_error(). No C++
+ *     exception crosses this boundary.
+ *   - Pointers documented as "device" point to HIP device memory owned by the
+ *     caller (the Python host hands in torch tensors' data_ptr()); the library
+ *     never frees caller memory. Internal scratch comes from HIP's
+ *     stream-ordered pool on the caller's stream and is returned before the
+ *     call ends.
+ *   - Work is ordered on `stream` (a hipStream_t, e.g. torch's current
+ *     stream). Entry points that report a size to the host (n_hits, n_out,
+ *     ...) synchronise the stream before returning.
+ *   - Output arenas have a capacity. When the result does not fit, the call
+ *     still returns FH_OK, reports the required size, and writes nothing
+ *     beyond the capacity; the caller grows the arena and calls again.
+ *   - Pair-list keys ("level keys") are uint64 values id * 4 + level, where id
+ *     is the reference's pair id (row_A * n_B + row_B for linkage,
+ *     row * n + col with row > col for deduplication) and level is 1 or 2.
+ *     Lists are ascending unless stated otherwise.
+ *
+ * Each declaration names the reference code (fast-er 0.3.1, paths relative to
+ * the reference repository root) whose role it takes over.
+ */
+#ifndef FASTER_HIP_H
+#define FASTER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* fh_stream_t; /* hipStream_t */
+
+enum {
+  FH_OK = 0,
+  FH_ERR_HIP = 1,         /* a HIP runtime call failed */
+  FH_ERR_ARG = 2,         /* invalid argument */
+  FH_ERR_UNSUPPORTED = 3, /* input outside what this build supports */
+  FH_ERR_INTERNAL = 4
+};
+
+enum { FH_MODE_LINK = 0, FH_MODE_DEDUP = 1 };
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fh_last_error(void);
+/* ABI version (major * 10000 + minor * 100 + patch). */
+int fh_version(void);
+
+/* Work accounting of one fh_jw_levels call (host struct, optional). */
+typedef struct {
+  int64_t pairs_logical;   /* value pairs the call covers */
+  int64_t pairs_kernel;    /* pairs evaluated by the bit-parallel kernel */
+  int64_t pairs_pruned;    /* pairs skipped because their lengths cannot reach level 1 */
+  int64_t pairs_generic;   /* pairs evaluated by the generic kernel (empty or > 64 bytes) */
+  double alg_lane_ops;     /* algorithmic VALU lane-ops of the kernel pairs */
+  int32_t alphabet;        /* byte values present on both sides */
+  int32_t launches;        /* kernel launches of the comparison stage */
+  double kernel_ms;        /* device time of the comparison kernels (HIP events) */
+} fh_jw_stats;
+
+/*
+ * All-pairs Jaro-Winkler levels over two sets of byte strings.
+ *
+ * Replaces the comparison kernel faster/comparison.py:133-186 (score of every
+ * pair, per-pair work arrays in global memory) together with the
+ * thresholding/compaction at faster/comparison.py:321-338; in
+ * FH_MODE_DEDUP it replaces faster/deduplication.py:132-196 and :513-529.
+ *
+ * Strings: chars_* (device bytes) with offsets_* (device int64, n + 1
+ * entries). Level 1: lower_thr <= s < upper_thr, level 2: s >= upper_thr, with
+ * s the reference's fp32 score (p is the prefix weight) and comparisons in
+ * fp32.
+ *   FH_MODE_LINK : every pair (a, b), string 1 = A value. Key
+ *                  (a * n_b + b) * 4 + level.
+ *   FH_MODE_DEDUP: set A only (chars_b/offsets_b/n_b ignored); every
+ *                  unordered pair of distinct indices x != y, string 1 = the
+ *                  lexicographically larger value. Key (x * n_a + y) * 4 +
+ *                  level with x the index of string 1. Equal indices (the
+ *                  reference's forced diagonal) are not emitted.
+ * hits: device uint64[capacity], unordered. *n_hits (host) receives the number
+ * of hits (possibly > capacity, see the conventions above).
+ */
+int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, int64_t n_a,
+                 const uint8_t* chars_b, const int64_t* offsets_b, int64_t n_b, int mode,
+                 float p, float lower_thr, float upper_thr, uint64_t* hits, int64_t capacity,
+                 int64_t* n_hits, fh_jw_stats* stats, fh_stream_t stream);
+
+/*
+ * Unique-value pair keys -> original-row pair keys (linkage).
+ *
+ * Replaces faster/comparison.py:429-453 (count gather, cumsum) and the
+ * expansion kernel faster/comparison.py:196-245. hits: keys
+ * (ua * n_ub + ub) * 4 + level. Unique value u of side A owns rows
+ * rows_a[starts_a[u] .. starts_a[u + 1]) (device int64). Output keys
+ * (row_a * n_b + row_b) * 4 + level, unordered; *n_out (host) = total.
+ */
+int fh_expand_link(const uint64_t* hits, int64_t n_hits, int64_t n_ub, const int64_t* starts_a,
+                   const int64_t* rows_a, const int64_t* starts_b, const int64_t* rows_b,
+                   int64_t n_b, uint64_t* out, int64_t capacity, int64_t* n_out,
+                   fh_stream_t stream);
+
+/*
+ * Unique-value pair keys -> original-row pair keys (deduplication).
+ *
+ * Replaces the count and expansion kernels faster/deduplication.py:206-243
+ * and :253-338 (with :550-580). hits: keys (x * n_u + y) * 4 + level over
+ * unique values; a pair of distinct values expands to every row pair, keyed
+ * max(r1, r2) * n + min(r1, r2). When diag_level > 0, every unique value
+ * with two or more rows also contributes its within-group pairs at that level
+ * (the reference's diagonal, whose score is forced to 1). Output unordered.
+ */
+int fh_expand_dedup(const uint64_t* hits, int64_t n_hits, int64_t n_u, const int64_t* starts,
+                    const int64_t* rows, int64_t n, int diag_level, uint64_t* out,
+                    int64_t capacity, int64_t* n_out, fh_stream_t stream);
+
+/*
+ * Exact-equality pairs (linkage), ascending.
+ *
+ * Replaces faster/comparison.py:500-600 (joint unique, the count == 2 test,
+ * the expansion kernel and the sort). group_a[i] (device int64) is the value
+ * id of A row i, or -1 when no B row holds the value; B rows holding value g
+ * are rows_b[starts_b[g] .. starts_b[g + 1]) in ascending order. Output keys
+ * (i * n_b + j) * 4 + 1, ascending.
+ */
+int fh_exact_link(const int64_t* group_a, int64_t n_a, const int64_t* starts_b,
+                  const int64_t* rows_b, int64_t n_groups, int64_t n_b, uint64_t* out,
+                  int64_t capacity, int64_t* n_out, fh_stream_t stream);
+
+/*
+ * Exact-equality pairs (deduplication), ascending.
+ *
+ * Replaces faster/deduplication.py:628-701 (output_count/repeat/triangular
+ * decode kernel :345-391 and the sort). group[i] is the value id of row i;
+ * rows of value g are rows[starts[g] .. starts[g + 1]) ascending. Output keys
+ * (i * n + j) * 4 + 1 for every j < i with an equal value, ascending.
+ */
+int fh_exact_dedup(const int64_t* group, int64_t n, const int64_t* starts, const int64_t* rows,
+                   int64_t n_groups, uint64_t* out, int64_t capacity, int64_t* n_out,
+                   fh_stream_t stream);
+
+/*
+ * Ascending sort of uint64 keys on bits [begin_bit, end_bit).
+ *
+ * Replaces cp.sort at faster/comparison.py:456,487,588 and
+ * faster/deduplication.py:577,614,689.
+ */
+int fh_sort_u64(uint64_t* keys, int64_t n, int begin_bit, int end_bit, fh_stream_t stream);
+
+/*
+ * Stable split of a sorted level-key list into per-level id lists.
+ *
+ * The reference keeps one list per level (faster/comparison.py:498). ids
+ * (device int64[n]) receives the ids of level 1 first, then level 2, ...,
+ * each ascending; level_offsets (host int64[n_levels + 1]) the boundaries,
+ * level 0 first (always empty).
+ */
+int fh_split_levels(const uint64_t* keys, int64_t n, int n_levels, int64_t* ids,
+                    int64_t* level_offsets, fh_stream_t stream);
+
+/*
+ * Membership mask of a sorted list in another sorted list.
+ *
+ * Replaces the binary search kernel faster/search.py:8-50 (inside = 1: the
+ * mask intersect() compacts with, inside = 0: the setdiff() mask).
+ */
+int fh_sorted_member_mask(const int64_t* arr1, int64_t n1, const int64_t* arr2, int64_t n2,
+                          int inside, uint8_t* mask, fh_stream_t stream);
+
+/*
+ * Pattern merge: per-variable level keys -> per-pattern id lists.
+ *
+ * Replaces the setdiff/intersect cascade faster/comparison.py:695-725
+ * (identical at faster/deduplication.py:773-803, with faster/search.py:57-157)
+ * and the list sizes behind Counts (faster/comparison.py:732-748).
+ * var_keys: host array of n_vars device pointers to ascending level-key lists
+ * (var_sizes host int64); var_levels: levels per variable (level 0 included).
+ * The pattern code of a pair is sum_v level_v * stride_v with the first
+ * variable slowest (itertools.product order). ids (device int64[capacity])
+ * receives the pairs of pattern 1, then pattern 2, ..., each ascending;
+ * pattern_offsets (host int64[n_patterns + 1]) the boundaries (pattern 0, the
+ * all-zero pattern, is empty). *n_union (host) = number of distinct pairs.
+ */
+int fh_pattern_merge(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
+                     const int32_t* var_levels, int64_t* ids, int64_t capacity,
+                     int64_t* pattern_offsets, int64_t* n_union, fh_stream_t stream);
+
+/*
+ * Fellegi-Sunter EM over a pattern histogram, FP64, one workgroup running
+ * every iteration on the device.
+ *
+ * Replaces Estimation.fit / _match_probability (faster/estimation.py:72-158).
+ * levels (host int32[n_vars]); counts (device int64[prod(levels)]) in
+ * itertools.product order. params (device double): in = initial values, out =
+ * final values, laid out [lambda, pi_0 of var 0..n-1 (levels each), pi_1 of
+ * var 0..n-1]. ksi (device double[n_patterns]) receives the match probability
+ * of every pattern under the final parameters. Stops when max |delta pi| <
+ * tol or after max_iter iterations; *iterations and *converged (host) report
+ * which.
+ */
+int fh_em_fit(int n_vars, const int32_t* levels, const int64_t* counts, double* params,
+              double tol, int max_iter, double* ksi, int* iterations, int* converged,
+              fh_stream_t stream);
+
+/* Match probability of every pattern under `params` (layout as above);
+ * faster/estimation.py:72-96 in FP64. */
+int fh_match_probability(int n_vars, const int32_t* levels, const double* params, double* ksi,
+                         fh_stream_t stream);
+
+/*
+ * Per-pair posterior gather: post[i] = ksi[k] for every pair i of pattern k,
+ * following the pattern-major layout of fh_pattern_merge
+ * (pattern_offsets device int64[n_patterns + 1]). The per-pair form of the
+ * selection at faster/linkage.py:48-54.
+ */
+int fh_pair_posteriors(const int64_t* pattern_offsets, int n_patterns, const double* ksi,
+                       double* post, int64_t n_pairs, fh_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTER_HIP_H */

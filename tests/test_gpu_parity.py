@@ -1,0 +1,224 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle.
+
+Integer/byte/index outputs must match bit for bit; EM parameters and
+posteriors within 1e-9 relative."""
+import json
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _np(t):
+    return t.detach().cpu().numpy().astype(np.int64) if isinstance(t, torch.Tensor) else np.asarray(t, np.int64)
+
+
+def _eq_lists(got, ref, what=""):
+    assert len(got) == len(ref), what
+    for k, (g, r) in enumerate(zip(got, ref)):
+        g, r = _np(g), np.asarray(r, dtype=np.int64)
+        assert g.shape == r.shape, f"{what}[{k}]: {g.shape} vs {r.shape}"
+        np.testing.assert_array_equal(g, r, err_msg=f"{what}[{k}]")
+
+
+def _words(rng, n, alpha="abcde", lo=0, hi=8, dup=None):
+    out = ["".join(alpha[k] for k in rng.integers(0, len(alpha), rng.integers(lo, hi + 1))) for _ in range(n)]
+    if dup:
+        for k in range(0, n, dup):
+            out[k] = out[(k * 7) % max(1, k)] if k else out[0]
+    return out
+
+
+def test_library_and_device():
+    import faster._lib as L
+
+    lib = L.load()
+    assert lib.fh_version() >= 100
+    assert torch.cuda.is_available()
+
+
+def test_kat_levels(oracle):
+    from faster.comparison import jaro_winkler_gpu
+
+    with open(os.path.join(HERE, "golden", "jw_kat.json")) as f:
+        kat = json.load(f)
+    A = [c["s1"] for c in kat["cases"]]
+    B = [c["s2"] for c in kat["cases"]]
+    for lo, hi in ((0.88, 0.94), (0.8, 0.9), (0.95, 0.99)):
+        got = jaro_winkler_gpu(A, B, 0, 0.1, lo, hi)
+        ref = oracle.jaro_winkler_all_pairs(A, B, 0, 0.1, lo, hi)
+        _eq_lists(got, ref, f"kat {lo},{hi}")
+
+
+@pytest.mark.parametrize("seed,alpha,lo_len,hi_len", [
+    (0, "abcde", 0, 8), (1, "ab", 1, 16), (2, "abcdefghijklmnopqrstuvwxyz", 2, 20),
+    (3, "abc", 14, 40), (4, "xyz", 30, 70), (5, "aAbB 0'-", 0, 24)])
+def test_all_pairs_levels(oracle, seed, alpha, lo_len, hi_len):
+    """Random strings over small alphabets (many matches), all length classes
+    (empty, 16-bit, 32-bit, 64-bit masks and the generic > 64 path)."""
+    from faster.comparison import jaro_winkler_gpu
+
+    rng = np.random.default_rng(seed)
+    A = _words(rng, 300, alpha, lo_len, hi_len)
+    B = _words(rng, 340, alpha, lo_len, hi_len)
+    for k in range(0, 300, 9):  # shared values and near copies
+        B[k] = A[k]
+    got = jaro_winkler_gpu(A, B, 17)
+    ref = oracle.jaro_winkler_all_pairs(A, B, 17)
+    _eq_lists(got, ref, f"seed {seed}")
+
+
+@pytest.mark.parametrize("lo,hi", [(0.88, 0.94), (0.7, 0.8), (0.94, 0.88), (0.0, 0.5), (0.99, 1.0)])
+def test_threshold_edge_cases(oracle, lo, hi):
+    """Thresholds at and below 0 (every pair is a hit), inverted, and 1.0."""
+    from faster.comparison import jaro_winkler_gpu
+
+    rng = np.random.default_rng(11)
+    A = _words(rng, 60, "abc", 0, 6)
+    B = _words(rng, 50, "abc", 0, 6)
+    _eq_lists(jaro_winkler_gpu(A, B, 0, 0.1, lo, hi), oracle.jaro_winkler_all_pairs(A, B, 0, 0.1, lo, hi),
+              f"thr {lo},{hi}")
+
+
+def test_unique_link(oracle):
+    from faster.comparison import jaro_winkler_unique_gpu
+
+    rng = np.random.default_rng(7)
+    A = _words(rng, 900, "abcdef", 1, 10)
+    B = _words(rng, 800, "abcdef", 1, 10)
+    for k in range(0, 800, 5):
+        B[k] = A[(k * 3) % 900]
+    for k in range(0, 900, 4):
+        A[k] = A[0] if k % 8 else A[k]
+    _eq_lists(jaro_winkler_unique_gpu(A, B), oracle.jaro_winkler_unique(A, B), "unique")
+
+
+def test_exact_link(oracle):
+    from faster.comparison import exact_gpu
+
+    rng = np.random.default_rng(8)
+    A = rng.integers(0, 50, 3000)
+    B = rng.integers(20, 80, 2500)
+    _eq_lists(exact_gpu(A, B), oracle.exact(A, B), "exact")
+    Sa = np.array(_words(rng, 500, "ab", 0, 3), dtype=object)
+    Sb = np.array(_words(rng, 400, "ab", 0, 3), dtype=object)
+    _eq_lists(exact_gpu(Sa, Sb), oracle.exact(Sa, Sb), "exact str")
+
+
+def test_sort_and_scan_primitives():
+    import faster._ops as ops
+
+    rng = np.random.default_rng(9)
+    for n in (0, 1, 5, 4095, 4096, 4097, 100_000, 1_500_000):
+        x = rng.integers(0, 2 ** 46, n, dtype=np.int64)
+        t = torch.from_numpy(x).cuda()
+        ops.sort_keys(t, 2 ** 46)
+        np.testing.assert_array_equal(t.cpu().numpy(), np.sort(x))
+
+
+def test_set_ops():
+    from faster.search import intersect, reduce, setdiff
+
+    rng = np.random.default_rng(10)
+    a = np.unique(rng.integers(0, 10 ** 6, 50_000))
+    b = np.unique(rng.integers(0, 10 ** 6, 70_000))
+    ta, tb = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    np.testing.assert_array_equal(_np(intersect(ta, tb)), np.intersect1d(a, b))
+    np.testing.assert_array_equal(_np(setdiff(ta, tb)), np.setdiff1d(a, b))
+    assert intersect(ta[:0], tb).dtype == torch.uint64
+    assert setdiff(ta[:0], tb) is not None
+    np.testing.assert_array_equal(_np(reduce(setdiff, [tb[:10], tb[10:20]], ta)),
+                                  np.setdiff1d(np.setdiff1d(a, b[:10]), b[10:20]))
+
+
+def _frames(n_a, n_b, seed):
+    from faster.synthetic import linkage_tables
+
+    return linkage_tables(n_a, n_b, seed=seed)
+
+
+def test_comparison_fit_small(oracle):
+    import faster
+
+    dfa, dfb, _ = _frames(1500, 1700, 3)
+    vf = ["first_name", "last_name", "street"]
+    comp = faster.Comparison(dfa, dfb, vf, vf, ["house_number"], ["house_number"])
+    comp.fit()
+    per_var, indices, counts = oracle.comparison_fit([dfa[v].to_numpy() for v in vf], [dfb[v].to_numpy() for v in vf],
+                                                     [dfa["house_number"].to_numpy()], [dfb["house_number"].to_numpy()])
+    for v in range(4):
+        _eq_lists(comp.Levels[v], per_var[v], f"var {v}")
+    _eq_lists(comp.Indices, indices, "Indices")
+    np.testing.assert_array_equal(comp.Counts, counts)
+    with pytest.raises(Exception, match="already been fitted"):
+        comp.fit()
+
+
+def test_deduplication_fit(oracle):
+    import faster
+    from faster.synthetic import dedup_table
+
+    df, _ = dedup_table(1200, seed=4)
+    df["first_name"] = [s[:3] for s in df["first_name"]]  # many identical values
+    vf = ["first_name", "last_name", "street"]
+    d = faster.Deduplication(df, vf, ["birth_year"])
+    d.fit()
+    per_var, indices, counts = oracle.deduplication_fit([df[v].to_numpy() for v in vf], [df["birth_year"].to_numpy()])
+    for v in range(4):
+        _eq_lists(d.Levels[v], per_var[v], f"dedup var {v}")
+    _eq_lists(d.Indices, indices, "dedup Indices")
+    np.testing.assert_array_equal(d.Counts, counts)
+
+
+def test_dedup_edge_values(oracle):
+    from faster.deduplication import exact_dedup_gpu, jaro_winkler_dedup_gpu
+
+    rng = np.random.default_rng(12)
+    S = _words(rng, 400, "ab", 0, 5) + ["a", "a", "", "", "x" * 70, "x" * 69 + "y", "x" * 70]
+    _eq_lists(jaro_winkler_dedup_gpu(S), oracle.jaro_winkler_dedup(S), "dedup edge")
+    E = rng.integers(0, 5, 700)
+    _eq_lists(exact_dedup_gpu(E), oracle.exact_dedup(E), "exact dedup")
+
+
+def test_em_matches_oracle(oracle):
+    import faster
+
+    rng = np.random.default_rng(13)
+    counts = np.concatenate([[10 ** 10], rng.integers(0, 10 ** 6, 53)]).astype(np.int64)
+    for tol, mi in ((1e-4, 5000), (1e-8, 5000), (1e-6, 3)):
+        np.random.seed(0)
+        est = faster.Estimation(3, 1, counts)
+        est.fit(Tolerance=tol, Max_Iter=mi)
+        np.random.seed(0)
+        lam, Pi, it, conv, ksi = oracle.em_fit([3, 3, 3, 2], counts, tol, mi)
+        assert est.Iterations == it
+        assert est.Lambda == pytest.approx(lam, rel=1e-9)
+        for m in range(2):
+            for k in range(4):
+                np.testing.assert_allclose(est.Pi[m][k], Pi[m][k], rtol=1e-9)
+        np.testing.assert_allclose(est.Ksi, ksi, rtol=1e-9, atol=1e-300)
+
+
+def test_pipeline_linkage_and_posteriors(oracle):
+    import faster
+
+    dfa, dfb, _ = _frames(800, 900, 5)
+    vf = ["first_name", "last_name", "street"]
+    comp = faster.Comparison(dfa, dfb, vf, vf, ["house_number"], ["house_number"])
+    comp.fit()
+    np.random.seed(0)
+    est = faster.Estimation(3, 1, comp.Counts)
+    est.fit()
+    link = faster.Linkage(dfa, dfb, comp.Indices, est.Ksi)
+    out = link.transform(0.85)
+    ra, rb = oracle.linkage_pairs([_np(x) for x in comp.Indices], est.Ksi, len(dfb), 0.85)
+    np.testing.assert_array_equal(out["Index_A"].to_numpy(), ra)
+    np.testing.assert_array_equal(out["Index_B"].to_numpy(), rb)
+    post = comp.posteriors(est.Ksi).cpu().numpy()
+    ref = np.concatenate([np.full(x.numel(), est.Ksi[k + 1]) for k, x in enumerate(comp.Indices)])
+    np.testing.assert_array_equal(post, ref)

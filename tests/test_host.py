@@ -1,0 +1,154 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every declared symbol, column encoding, threshold semantics, class
+validation and error paths."""
+import ctypes
+import os
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pandas as pd
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    with open(os.path.join(ROOT, "include", "faster_hip.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+char\s*\*|int)\s+(fh_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    import faster._lib as L
+
+    lib = L.load(require_gpu=False)
+    decl = _declared_symbols()
+    assert len(decl) >= 14
+    for name in decl:
+        assert hasattr(lib, name), name
+    assert set(decl) == set(L.EXPORTED)
+    assert lib.fh_version() == 100
+    assert lib.fh_last_error() == b""
+
+
+def test_ops_fail_loudly_without_gpu():
+    import torch
+
+    import faster._lib as L
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(RuntimeError, match="needs a ROCm GPU"):
+        L.load()
+
+
+def test_group_strings_matches_numpy_unique():
+    from faster.encode import group_strings
+
+    rng = np.random.default_rng(0)
+    vals = ["".join("abé"[k] for k in rng.integers(0, 3, rng.integers(0, 6))) for _ in range(500)]
+    g = group_strings(vals)
+    u, inv, cnt = np.unique(np.array(vals, dtype=object), return_inverse=True, return_counts=True)
+    assert g.n_unique == len(u)
+    for k in range(len(u)):
+        b = g.chars[g.offsets[k]:g.offsets[k + 1]].tobytes().decode("utf-8")
+        assert b == u[k]
+        rows = g.rows[g.starts[k]:g.starts[k + 1]]
+        np.testing.assert_array_equal(rows, np.nonzero(np.asarray(inv).reshape(-1) == k)[0])
+    g2 = group_strings(["a\x00b", "a", "a\x00b"])
+    assert g2.n_unique == 2
+
+
+def test_group_exact():
+    from faster.encode import group_exact_dedup, group_exact_link
+
+    a = np.array([3, 1, 4, 1, 5])
+    b = np.array([1, 5, 9, 1])
+    g = group_exact_link(a, b)
+    pairs = []
+    for i in range(len(a)):
+        gid = g.group_a[i]
+        for j in g.rows_b[g.starts_b[gid]:g.starts_b[gid + 1]]:
+            pairs.append(i * len(b) + j)
+    ref = [i * len(b) + j for i in range(len(a)) for j in range(len(b)) if a[i] == b[j]]
+    assert sorted(pairs) == ref
+    d = group_exact_dedup(a)
+    assert d.n_groups == 4
+    assert list(d.rows[d.starts[d.group[1]]:d.starts[d.group[1] + 1]]) == [1, 3]
+
+
+def test_threshold_semantics_fp32():
+    from faster.levels import level_of_score
+
+    f88 = float(np.float32(0.88))
+    assert level_of_score(f88, 0.88, 0.94) == 1
+    assert level_of_score(np.nextafter(np.float32(f88), np.float32(0)), 0.88, 0.94) == 0
+    assert level_of_score(1.0, 0.88, 0.94) == 2
+    assert level_of_score(0.0, 0.0, 0.5) == 1
+    assert level_of_score(0.9, 0.94, 0.88) == 2
+
+
+def test_class_validation_and_errors():
+    import faster
+
+    assert faster.__version__ == "0.3.1"
+    df = pd.DataFrame({"a": ["x"], "b": ["y"], "n": [1]})
+    with pytest.raises(Exception, match="lengths of Vars_Fuzzy_A and Vars_Fuzzy_B differ"):
+        faster.Comparison(df, df, ["a"], [])
+    with pytest.raises(Exception, match="lengths of Vars_Exact_A and Vars_Exact_B differ"):
+        faster.Comparison(df, df, ["a"], ["a"], ["n"], [])
+    with pytest.raises(Exception, match="Vars_Fuzzy_A and Vars_Fuzzy_B must match"):
+        faster.Comparison(df, df, ["zz"], ["a"])
+    with pytest.raises(Exception, match="Vars_Exact_A and Vars_Exact_B must match"):
+        faster.Comparison(df, df, ["a"], ["a"], ["zz"], ["n"])
+    with pytest.raises(Exception, match="must match variable names in df"):
+        faster.Deduplication(df, ["zz"])
+    c = faster.Comparison(df, df, ["a"], ["a"])
+    with pytest.raises(Exception, match="must be fitted first"):
+        c.Counts
+    d = faster.Deduplication(df, ["a"])
+    with pytest.raises(Exception, match="must be fitted first"):
+        d.Counts
+    e = faster.Estimation(3, 1, np.ones(54, dtype=np.int64))
+    with pytest.raises(Exception, match="must be fitted first"):
+        e.Ksi
+    import itertools
+
+    np.testing.assert_array_equal(e.Gamma, np.array(list(itertools.product(range(3), range(3), range(3), range(2)))))
+
+
+def test_evaluation_host_math():
+    import faster
+
+    ksi = np.array([0.01, 0.5, 0.99])
+    counts = np.array([100, 10, 5])
+    ev = faster.Evaluation(0.05, ksi, counts)
+    assert ev.FDR(0.9) == pytest.approx(0.01)
+    assert ev.FNR(0.9) == pytest.approx(np.sum(ksi * (ksi < 0.9) * counts) / 0.05 * counts.sum())
+    assert ev.FNR_rate(0.9) == pytest.approx(np.sum(ksi * (ksi < 0.9) * counts) / (0.05 * counts.sum()))
+    assert ev.FDR(1.1) is None
+    t = ev.Optimal_Threshold(1.0)
+    assert 0.0 <= t <= 1.0
+
+
+def test_synthetic_deterministic():
+    from faster.synthetic import dedup_table, linkage_tables
+
+    a1, b1, t1 = linkage_tables(200, 300, seed=1)
+    a2, b2, t2 = linkage_tables(200, 300, seed=1)
+    pd.testing.assert_frame_equal(a1, a2)
+    pd.testing.assert_frame_equal(b1, b2)
+    assert (t1 >= 0).sum() == 90
+    lens = a1["first_name"].str.len()
+    assert lens.min() >= 2 and lens.max() <= 20
+    d, cl = dedup_table(500, seed=2)
+    assert (cl >= 0).sum() >= 90
+
+
+def test_bench_refuses_multi_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
