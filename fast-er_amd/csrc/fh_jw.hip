@@ -280,10 +280,10 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
       const uint32_t h = (~(uint32_t)window_bits(i, c.w, lenB)) & 0xFFFFu;
       nwin[i] = __builtin_amdgcn_readfirstlane(h | (h << 16));
     }
-    const uint32_t h = (0xFFFFu << lenB) & 0xFFFFu;
-    const uint32_t init = h | (h << 16);
-    const int thr = 16 - c.mm;   // popcount of a half <= thr  <=>  m >= mm
-    const int quick = thr + 17;  // popcount of the dword >= quick  =>  neither half survives
+    // navail starts at 0: positions past lenB never become candidates because
+    // the tile masks and the windows block them, so popcount(navail) = m.
+    const uint32_t init = 0u;
+    const int mm = c.mm;
     const int ncols = (c.cntB + 1) >> 1;
     for (int col = 0; col < ncols; col += kU) {
       uint32_t na[kU];
@@ -298,15 +298,16 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
           na[u] = select_bits(nc, na[u], add1_u16x2(nc));
         }
       }
-      int mn = 64;
+      // A half reaches mm only if its dword does.
+      int mx = 0;
 #pragma unroll
-      for (int u = 0; u < kU; ++u) mn = min(mn, (int)__popc(na[u]));
-      if (__ballot(mn < quick && c.lane_ok) == 0) continue;
+      for (int u = 0; u < kU; ++u) mx = max(mx, (int)__popc(na[u]));
+      if (__ballot(mx >= mm && c.lane_ok) == 0) continue;
       uint32_t smask = 0;
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        smask |= (uint32_t)((int)__popc(na[u] & 0xFFFFu) <= thr) << (2 * u);
-        smask |= (uint32_t)((int)__popc(na[u] >> 16) <= thr) << (2 * u + 1);
+        smask |= (uint32_t)((int)__popc(na[u] & 0xFFFFu) >= mm) << (2 * u);
+        smask |= (uint32_t)((int)__popc(na[u] >> 16) >= mm) << (2 * u + 1);
       }
       push_candidates(a, c, smask, 2 * col, 2 * kU);
     }
@@ -314,8 +315,8 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
     uint32_t nwin[LA];
 #pragma unroll
     for (int i = 0; i < LA; ++i) nwin[i] = __builtin_amdgcn_readfirstlane(~(uint32_t)window_bits(i, c.w, lenB));
-    const uint32_t init = lenB >= 32 ? 0u : (0xFFFFFFFFu << lenB);
-    const int thr = 32 - c.mm;
+    const uint32_t init = 0u;
+    const int mm = c.mm;
     for (int col = 0; col < c.cntB; col += kU) {
       uint32_t na[kU];
 #pragma unroll
@@ -331,7 +332,7 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
       }
       uint32_t smask = 0;
 #pragma unroll
-      for (int u = 0; u < kU; ++u) smask |= (uint32_t)((int)__popc(na[u]) <= thr) << u;
+      for (int u = 0; u < kU; ++u) smask |= (uint32_t)((int)__popc(na[u]) >= mm) << u;
       if (__ballot(smask != 0 && c.lane_ok) == 0) continue;
       push_candidates(a, c, smask, col, kU);
     }
@@ -349,15 +350,7 @@ __device__ __forceinline__ void tile_pass_rolled(const JwArgs& a, PassCtx& c, co
   const int step = KIND == kW64 ? 2 : 1;
 #pragma unroll 1
   for (int col = 0; col < ncols; col += step) {
-    uint64_t na;
-    if (KIND == kP16) {
-      const uint32_t h = (0xFFFFu << lenB) & 0xFFFFu;
-      na = h | (h << 16);
-    } else if (KIND == kW32) {
-      na = lenB >= 32 ? 0u : (0xFFFFFFFFu << lenB);
-    } else {
-      na = lenB >= 64 ? 0ull : (~0ull << lenB);
-    }
+    uint64_t na = 0;  // popcount(na) = m (see tile_pass)
 #pragma unroll 1
     for (int i = 0; i < lenA; ++i) {
       uint32_t oi = 0;
@@ -385,12 +378,12 @@ __device__ __forceinline__ void tile_pass_rolled(const JwArgs& a, PassCtx& c, co
     }
     uint32_t smask = 0;
     if (KIND == kP16) {
-      smask = (uint32_t)((int)__popc((uint32_t)na & 0xFFFFu) <= 16 - c.mm) |
-              ((uint32_t)((int)__popc((uint32_t)na >> 16) <= 16 - c.mm) << 1);
+      smask = (uint32_t)((int)__popc((uint32_t)na & 0xFFFFu) >= c.mm) |
+              ((uint32_t)((int)__popc((uint32_t)na >> 16) >= c.mm) << 1);
     } else if (KIND == kW32) {
-      smask = (uint32_t)((int)__popc((uint32_t)na) <= 32 - c.mm);
+      smask = (uint32_t)((int)__popc((uint32_t)na) >= c.mm);
     } else {
-      smask = (uint32_t)((int)__popcll(na) <= 64 - c.mm);
+      smask = (uint32_t)((int)__popcll(na) >= c.mm);
     }
     if (__ballot(smask != 0 && c.lane_ok) == 0) continue;
     push_candidates(a, c, smask, col * per / dws, per);
