@@ -298,11 +298,15 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
           na[u] = select_bits(nc, na[u], add1_u16x2(nc));
         }
       }
-      // A half reaches mm only if its dword does.
-      int mx = 0;
+      // Largest match count over the 2 * kU pairs: popcount of each half.
+      uint32_t mx = 0;
 #pragma unroll
-      for (int u = 0; u < kU; ++u) mx = max(mx, (int)__popc(na[u]));
-      if (__ballot(mx >= mm && c.lane_ok) == 0) continue;
+      for (int u = 0; u < kU; ++u) {
+        const uint32_t lo = (uint32_t)__popc(na[u] & 0xFFFFu);
+        const uint32_t hi = (uint32_t)__popc(na[u] >> 16);
+        mx = max(mx, max(lo, hi));
+      }
+      if (__ballot((int)mx >= mm && c.lane_ok) == 0) continue;
       uint32_t smask = 0;
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
