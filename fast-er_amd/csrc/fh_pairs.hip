@@ -232,6 +232,13 @@ struct MergeArgs {
   int64_t sizes[kMaxVars];
   uint32_t stride[kMaxVars];
   int n_vars;
+  // Exact variables given as per-row value ids (level = ids equal): pair id
+  // -> (id / id_div, id % id_div); row values imp_a[k][row1], imp_b[k][row2].
+  const int64_t* imp_a[kMaxVars];
+  const int64_t* imp_b[kMaxVars];
+  uint32_t imp_stride[kMaxVars];
+  int n_imp;
+  uint64_t id_div;
 };
 
 __device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* a, int64_t lo, int64_t hi, uint64_t x) {
@@ -306,6 +313,15 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
       }
     }
     __syncthreads();
+  }
+  if (m.n_imp > 0) {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      if (!owner[k]) continue;
+      const uint64_t r1 = id[k] / m.id_div, r2 = id[k] % m.id_div;
+      for (int q = 0; q < m.n_imp; ++q)
+        if (m.imp_a[q][r1] == m.imp_b[q][r2]) code[k] += m.imp_stride[q];
+    }
   }
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
@@ -534,15 +550,17 @@ extern "C" int fh_sorted_member_mask(const int64_t* arr1, int64_t n1, const int6
   return FH_OK;
 }
 
-extern "C" int fh_pattern_merge(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
-                                const int32_t* var_levels, int64_t* ids, int64_t capacity, int64_t* pattern_offsets,
-                                int64_t* n_union, fh_stream_t stream) {
-  clear_error();
-  FH_REQUIRE(n_vars >= 1 && n_vars <= kMaxVars, "fh_pattern_merge: 1..%d variables supported", kMaxVars);
+static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
+                              const int32_t* var_levels, int n_imp, const int64_t* const* imp_a,
+                              const int64_t* const* imp_b, int64_t id_div, int64_t* ids, int64_t capacity,
+                              int64_t* pattern_offsets, int64_t* n_union, hipStream_t s) {
+  FH_REQUIRE(n_vars >= 1 && n_vars <= kMaxVars, "fh_pattern_merge: 1..%d listed variables supported", kMaxVars);
+  FH_REQUIRE(n_imp >= 0 && n_imp <= kMaxVars, "fh_pattern_merge: 0..%d row-id variables supported", kMaxVars);
   FH_REQUIRE(pattern_offsets != nullptr && n_union != nullptr, "fh_pattern_merge: null host outputs");
-  hipStream_t s = (hipStream_t)stream;
   MergeArgs m;
   m.n_vars = n_vars;
+  m.n_imp = n_imp;
+  m.id_div = (uint64_t)(id_div > 0 ? id_div : 1);
   uint64_t n_patterns = 1;
   for (int v = 0; v < n_vars; ++v) {
     FH_REQUIRE(var_levels[v] >= 2 && var_levels[v] <= 4, "fh_pattern_merge: levels must be 2..4");
@@ -550,11 +568,22 @@ extern "C" int fh_pattern_merge(int n_vars, const uint64_t* const* var_keys, con
     n_patterns *= (uint64_t)var_levels[v];
     FH_REQUIRE(n_patterns < (1ull << 31), "fh_pattern_merge: too many patterns");
   }
-  for (int v = n_vars - 1, st = 1; v >= 0; --v) {
-    m.stride[v] = (uint32_t)st;
-    m.lists[v] = var_keys[v];
-    m.sizes[v] = var_sizes[v];
-    st *= var_levels[v];
+  n_patterns <<= n_imp;
+  FH_REQUIRE(n_patterns < (1ull << 31), "fh_pattern_merge: too many patterns");
+  {
+    uint32_t st = 1;
+    for (int q = n_imp - 1; q >= 0; --q) {
+      m.imp_stride[q] = st;
+      m.imp_a[q] = imp_a[q];
+      m.imp_b[q] = imp_b[q];
+      st *= 2;
+    }
+    for (int v = n_vars - 1; v >= 0; --v) {
+      m.stride[v] = st;
+      m.lists[v] = var_keys[v];
+      m.sizes[v] = var_sizes[v];
+      st *= (uint32_t)var_levels[v];
+    }
   }
   for (uint64_t c = 0; c <= n_patterns; ++c) pattern_offsets[c] = 0;
   Scratch scr(s);
@@ -613,6 +642,25 @@ extern "C" int fh_pattern_merge(int n_vars, const uint64_t* const* var_keys, con
   pattern_offsets[n_patterns] = base;
   *n_union = base;
   return FH_OK;
+}
+
+extern "C" int fh_pattern_merge(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
+                                const int32_t* var_levels, int64_t* ids, int64_t capacity, int64_t* pattern_offsets,
+                                int64_t* n_union, fh_stream_t stream) {
+  clear_error();
+  return pattern_merge_impl(n_vars, var_keys, var_sizes, var_levels, 0, nullptr, nullptr, 1, ids, capacity,
+                            pattern_offsets, n_union, (hipStream_t)stream);
+}
+
+extern "C" int fh_pattern_merge_rowids(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
+                                       const int32_t* var_levels, int n_exact, const int64_t* const* ids_a,
+                                       const int64_t* const* ids_b, int64_t id_div, int64_t* ids,
+                                       int64_t capacity, int64_t* pattern_offsets, int64_t* n_union,
+                                       fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(id_div > 0, "fh_pattern_merge_rowids: id_div must be positive");
+  return pattern_merge_impl(n_vars, var_keys, var_sizes, var_levels, n_exact, ids_a, ids_b, id_div, ids, capacity,
+                            pattern_offsets, n_union, (hipStream_t)stream);
 }
 
 extern "C" int fh_pair_posteriors(const int64_t* pattern_offsets, int n_patterns, const double* ksi, double* post,

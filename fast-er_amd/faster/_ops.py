@@ -157,21 +157,32 @@ def member_mask(arr1: torch.Tensor, arr2: torch.Tensor, inside: bool) -> torch.T
     return mask[:n1].bool()
 
 
-def pattern_merge(var_keys, var_levels):
-    """Per-variable sorted level keys -> (ids, host pattern offsets, n_union)."""
+def pattern_merge(var_keys, var_levels, rowid_vars=None, id_div=1):
+    """Per-variable sorted level keys -> (ids, host pattern offsets, n_union).
+
+    ``rowid_vars``: optional list of (ids_a, ids_b) device int64 tensors of
+    two-level exact variables that follow the listed ones in pattern order and
+    are looked up per pair instead of being listed (fh_pattern_merge_rowids)."""
     L = _lib.load()
     nv = len(var_keys)
     keys = [k.contiguous() for k in var_keys]
     ptrs = (ctypes.c_void_p * nv)(*[k.data_ptr() if k.numel() else 0 for k in keys])
     sizes = (ctypes.c_int64 * nv)(*[k.numel() for k in keys])
     levels = (ctypes.c_int32 * nv)(*[int(x) for x in var_levels])
-    n_patterns = int(np.prod([int(x) for x in var_levels]))
+    ne = len(rowid_vars) if rowid_vars else 0
+    n_patterns = int(np.prod([int(x) for x in var_levels])) * (2 ** ne)
     cap = max(1, sum(k.numel() for k in keys))
     ids = _empty_i64(cap)
     offs = (ctypes.c_int64 * (n_patterns + 1))()
     n_union = ctypes.c_int64(0)
-    check(L.fh_pattern_merge(nv, ptrs, sizes, levels, ptr(ids), _i64(cap), offs, ctypes.byref(n_union),
-                             stream()), "fh_pattern_merge")
+    if ne:
+        pa = (ctypes.c_void_p * ne)(*[a.data_ptr() for a, _ in rowid_vars])
+        pb = (ctypes.c_void_p * ne)(*[b.data_ptr() for _, b in rowid_vars])
+        check(L.fh_pattern_merge_rowids(nv, ptrs, sizes, levels, ne, pa, pb, _i64(id_div), ptr(ids), _i64(cap),
+                                        offs, ctypes.byref(n_union), stream()), "fh_pattern_merge_rowids")
+    else:
+        check(L.fh_pattern_merge(nv, ptrs, sizes, levels, ptr(ids), _i64(cap), offs, ctypes.byref(n_union),
+                                 stream()), "fh_pattern_merge")
     if n_union.value > cap:
         raise RuntimeError("pattern merge: union larger than the sum of its inputs")
     return ids[:n_union.value], [offs[i] for i in range(n_patterns + 1)], n_union.value

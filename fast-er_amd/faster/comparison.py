@@ -16,7 +16,7 @@ import numpy as np
 import pandas as pd
 import torch
 
-from . import _lib, _ops
+from . import _lib, _ops, patterns
 from .encode import group_exact_link, group_strings, pack_strings
 from .levels import level_of_score
 
@@ -128,13 +128,21 @@ class Comparison():
         self._Fit_flag = False
         self._var_keys = None
 
-    def fit(self, p=0.1, Lower_Thr=0.88, Upper_Thr=0.94, Num_Threads=256, Max_Chunk_Size=2.0):
+    def fit(self, p=0.1, Lower_Thr=0.88, Upper_Thr=0.94, Num_Threads=256, Max_Chunk_Size=2.0, *,
+            Exact_Mode="auto"):
         """Compares all pairs of observations on the selected variables; the
-        result is stored in ``Indices`` (faster/comparison.py:662-730)."""
+        result is stored in ``Indices`` (faster/comparison.py:662-730).
+
+        ``Exact_Mode`` (keyword-only): 'list' materialises every exact
+        variable's equal pairs as the reference does; 'rowid' keeps exact
+        variables as per-row value ids, counts the patterns without a fuzzy
+        agreement in closed form and leaves their lists lazy; 'auto' picks
+        'rowid' when an exact variable has more than
+        ``patterns.ROWID_EXACT_MIN_PAIRS`` equal pairs."""
         if self._Fit_flag:
             raise Exception("If the model has already been fitted, it cannot be fitted again.")
         _lib.load()
-        n_b = len(self.df_B)
+        n_a, n_b = len(self.df_A), len(self.df_B)
         stats = []
         keys = []
         for va, vb in zip(self.Vars_Fuzzy_A, self.Vars_Fuzzy_B):
@@ -142,21 +150,50 @@ class Comparison():
             gb = group_strings(self.df_B[vb].to_numpy()).to_device()
             keys.append(fuzzy_keys_link(ga, gb, p, Lower_Thr, Upper_Thr, stats))
             del ga, gb
-        for va, vb in zip(self.Vars_Exact_A, self.Vars_Exact_B):
-            g = group_exact_link(self.df_A[va].to_numpy(), self.df_B[vb].to_numpy())
-            ga, sb, rb = g.to_device()
-            keys.append(exact_keys_link(ga, sb, rb, g.n_groups, n_b))
-        self._finish(keys, len(self.df_A) * n_b)
+        exact = [group_exact_link(self.df_A[va].to_numpy(), self.df_B[vb].to_numpy())
+                 for va, vb in zip(self.Vars_Exact_A, self.Vars_Exact_B)]
+        totals = []
+        for g in exact:
+            ca = np.bincount(g.group_a, minlength=g.n_groups).astype(np.int64)
+            totals.append(int(np.dot(ca, np.diff(g.starts_b))))
+        mode = patterns.choose_exact_mode(Exact_Mode, len(self.Vars_Fuzzy_A), totals)
+        self.Exact_Mode = mode
         self.Stats = stats
+        lv_f = [_LEVELS_FUZZY] * len(self.Vars_Fuzzy_A)
+        self._var_levels = lv_f + [_LEVELS_EXACT] * len(exact)
+        self._n_pairs = n_a * n_b
+        if mode == "rowid":
+            dev = [g.to_device() for g in exact]
+            dev_b = [torch.from_numpy(g.group_b).to(_lib.device()) for g in exact]
+            rowid_dev = [(d[0], b) for d, b in zip(dev, dev_b)]
+            rowid_host = [(g.group_a, g.group_b) for g in exact]
+
+            def mat(d, ids, offs):
+                if len(exact) != 1:
+                    raise NotImplementedError("materialising exact-only patterns needs a single exact variable")
+                eq = _split(exact_keys_link(dev[0][0], dev[0][1], dev[0][2], exact[0].n_groups, n_b), 2)[0]
+                return patterns.materialize_exact_only(eq, ids, offs, d, 2)
+
+            ids, offs, self.Indices, self._Counts = patterns.merge_with_rowids(
+                keys, lv_f, rowid_dev, rowid_host, n_b, False, n_a * n_b, mat)
+            self._pattern_ids = ids
+            self._pattern_offsets = offs
+            self._var_keys = keys
+            self._rowid_exact = (exact, dev, totals)
+            self._Fit_flag = True
+        else:
+            for g in exact:
+                ga, sb, rb = g.to_device()
+                keys.append(exact_keys_link(ga, sb, rb, g.n_groups, n_b))
+            self._rowid_exact = None
+            self._finish(keys, n_a * n_b)
 
     def _finish(self, keys, n_pairs):
-        levels = [_LEVELS_FUZZY] * len(self.Vars_Fuzzy_A) + [_LEVELS_EXACT] * len(self.Vars_Exact_A)
-        ids, offs, n_union = _ops.pattern_merge(keys, levels)
+        ids, offs, n_union = _ops.pattern_merge(keys, self._var_levels)
         self._pattern_ids = ids
         self._pattern_offsets = offs
         self.Indices = [ids[offs[k]:offs[k + 1]] for k in range(1, len(offs) - 1)]
         self._var_keys = keys
-        self._var_levels = levels
         self._n_pairs = n_pairs
         self._Fit_flag = True
 
@@ -176,13 +213,21 @@ class Comparison():
     @property
     def Levels(self):
         """Per-variable, per-level pair lists: ``Levels[v][l - 1]`` holds the
-        ascending ids at level ``l`` of variable ``v`` (fuzzy variables first)."""
+        ascending ids at level ``l`` of variable ``v`` (fuzzy variables first;
+        exact variables kept as row ids appear as ``patterns.RowIdPairs``)."""
         if not self._Fit_flag:
             raise Exception("The model must be fitted first.")
         try:
             return self._Levels
         except AttributeError:
-            self._Levels = [_split(k, L) for k, L in zip(self._var_keys, self._var_levels)]
+            out = [_split(k, L) for k, L in zip(self._var_keys, self._var_levels)]
+            if self._rowid_exact is not None:
+                exact, dev, totals = self._rowid_exact
+                n_b = len(self.df_B)
+                for g, d, t in zip(exact, dev, totals):
+                    out.append([patterns.RowIdPairs(t, lambda g=g, d=d: _split(
+                        exact_keys_link(d[0], d[1], d[2], g.n_groups, n_b), 2)[0])])
+            self._Levels = out
             return self._Levels
 
     def posteriors(self, Ksi):

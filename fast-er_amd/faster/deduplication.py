@@ -14,7 +14,7 @@ import numpy as np
 import pandas as pd
 import torch
 
-from . import _lib, _ops
+from . import _lib, _ops, patterns
 from .comparison import _split
 from .encode import group_exact_dedup, group_strings
 from .levels import level_of_score
@@ -79,29 +79,58 @@ class Deduplication():
         self.Stats = None
         self._Fit_flag = False
 
-    def fit(self, p=0.1, Lower_Thr=0.88, Upper_Thr=0.94, Num_Threads=256, Max_Chunk_Size=2.0):
+    def fit(self, p=0.1, Lower_Thr=0.88, Upper_Thr=0.94, Num_Threads=256, Max_Chunk_Size=2.0, *,
+            Exact_Mode="auto"):
         """Compares all pairs of observations; the result is stored in ``Indices``
-        (faster/deduplication.py:740-808)."""
+        (faster/deduplication.py:740-808).
+
+        ``Exact_Mode`` (keyword-only): 'list' materialises every exact variable's
+        equal pairs as the reference does; 'rowid' keeps exact variables as
+        per-row value ids, counts the patterns without a fuzzy agreement in
+        closed form and leaves their lists lazy (``patterns.RowIdPairs``);
+        'auto' picks 'rowid' when an exact variable has more than
+        ``patterns.ROWID_EXACT_MIN_PAIRS`` equal pairs."""
         if self._Fit_flag:
             raise Exception("If the model has already been fitted, it cannot be fitted again.")
         _lib.load()
+        n = len(self.df)
         stats = []
         keys = []
         for v in self.Vars_Fuzzy:
             g = group_strings(self.df[v].to_numpy()).to_device()
             keys.append(fuzzy_keys_dedup(g, p, Lower_Thr, Upper_Thr, stats))
             del g
-        for v in self.Vars_Exact:
-            g = group_exact_dedup(self.df[v].to_numpy())
-            grp, st, rows = g.to_device()
-            keys.append(_ops.exact_dedup(grp, st, rows, g.n_groups))
-        levels = [_LEVELS_FUZZY] * len(self.Vars_Fuzzy) + [_LEVELS_EXACT] * len(self.Vars_Exact)
-        ids, offs, _ = _ops.pattern_merge(keys, levels)
+        exact = [group_exact_dedup(self.df[v].to_numpy()) for v in self.Vars_Exact]
+        totals = [int(np.sum(np.diff(g.starts) * (np.diff(g.starts) - 1) // 2)) for g in exact]
+        mode = patterns.choose_exact_mode(Exact_Mode, len(self.Vars_Fuzzy), totals)
+        self.Exact_Mode = mode
+        lv_f = [_LEVELS_FUZZY] * len(self.Vars_Fuzzy)
+        if mode == "rowid":
+            dev = [g.to_device() for g in exact]
+            rowid_dev = [(d[0], d[0]) for d in dev]
+            rowid_host = [(g.group, g.group) for g in exact]
+
+            def mat(d, ids, offs):
+                if len(exact) != 1:
+                    raise NotImplementedError("materialising exact-only patterns needs a single exact variable")
+                grp, st, rows = dev[0]
+                eq = _split(_ops.exact_dedup(grp, st, rows, exact[0].n_groups), 2)[0]
+                return patterns.materialize_exact_only(eq, ids, offs, d, 2)
+
+            ids, offs, self.Indices, self._Counts = patterns.merge_with_rowids(
+                keys, lv_f, rowid_dev, rowid_host, n, True, n * (n + 1) // 2, mat)
+            self._rowid_exact = (exact, dev, totals)
+        else:
+            for g in exact:
+                grp, st, rows = g.to_device()
+                keys.append(_ops.exact_dedup(grp, st, rows, g.n_groups))
+            ids, offs, _ = _ops.pattern_merge(keys, lv_f + [_LEVELS_EXACT] * len(exact))
+            self.Indices = [ids[offs[k]:offs[k + 1]] for k in range(1, len(offs) - 1)]
+            self._rowid_exact = None
         self._pattern_ids = ids
         self._pattern_offsets = offs
-        self.Indices = [ids[offs[k]:offs[k + 1]] for k in range(1, len(offs) - 1)]
         self._var_keys = keys
-        self._var_levels = levels
+        self._var_levels = lv_f + [_LEVELS_EXACT] * len(exact)
         self.Stats = stats
         self._Fit_flag = True
 
@@ -122,11 +151,18 @@ class Deduplication():
 
     @property
     def Levels(self):
-        """Per-variable, per-level pair lists (fuzzy variables first)."""
+        """Per-variable, per-level pair lists (fuzzy variables first; exact
+        variables kept as row ids appear as ``patterns.RowIdPairs``)."""
         if not self._Fit_flag:
             raise Exception("The model must be fitted first.")
         try:
             return self._Levels
         except AttributeError:
-            self._Levels = [_split(k, L) for k, L in zip(self._var_keys, self._var_levels)]
+            out = [_split(k, L) for k, L in zip(self._var_keys, self._var_levels)]
+            if self._rowid_exact is not None:
+                exact, dev, totals = self._rowid_exact
+                for g, d, t in zip(exact, dev, totals):
+                    out.append([patterns.RowIdPairs(t, lambda g=g, d=d: _split(
+                        _ops.exact_dedup(d[0], d[1], d[2], g.n_groups), 2)[0])])
+            self._Levels = out
             return self._Levels

@@ -116,6 +116,7 @@ class ExactLinkGroups:
     starts_b: np.ndarray  # int64[n_groups + 1]
     rows_b: np.ndarray    # int64[n_B]: B rows of each joint value, ascending
     n_groups: int
+    group_b: np.ndarray = None  # int64[n_B]: joint value id of each B row
 
     def to_device(self):
         dev = _lib.device()
@@ -137,7 +138,7 @@ def group_exact_link(values_a, values_b) -> ExactLinkGroups:
     starts_b = np.zeros(n_groups + 1, dtype=np.int64)
     np.cumsum(cnt_b, out=starts_b[1:])
     rows_b = np.argsort(inv_b, kind="stable").astype(np.int64)
-    return ExactLinkGroups(inv_a.copy(), starts_b, rows_b, n_groups)
+    return ExactLinkGroups(inv_a.copy(), starts_b, rows_b, n_groups, inv_b.copy())
 
 
 @dataclass

@@ -40,8 +40,12 @@ class Linkage():
         if np.sum([_numel(self.Indices[i - 1]) for i in above]) == 0:
             raise Exception("No pair of observations has a conditional match probability exceeding the threshold.")
         dev = _lib.device()
-        parts = [x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x, dtype=np.int64))
-                 for x in (self.Indices[i - 1] for i in above)]
+        parts = []
+        for i in above:
+            x = self.Indices[i - 1]
+            if hasattr(x, "materialize"):
+                x = x.materialize()
+            parts.append(x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x, dtype=np.int64)))
         ids = torch.cat([x.to(dev, torch.int64) for x in parts])
         n_b = len(df_B)
         a = (ids // n_b).cpu().numpy()

@@ -190,6 +190,25 @@ int fh_pattern_merge(int n_vars, const uint64_t* const* var_keys, const int64_t*
                      int64_t* pattern_offsets, int64_t* n_union, fh_stream_t stream);
 
 /*
+ * Pattern merge with exact variables given as per-row value ids.
+ *
+ * As fh_pattern_merge for the n_vars listed variables, followed in pattern
+ * order by n_exact two-level exact variables that are not materialised as
+ * pair lists: the level of pair id is 1 when ids_a[k][id / id_div] ==
+ * ids_b[k][id % id_div] (device int64 arrays; for deduplication ids_b[k] =
+ * ids_a[k] and id_div = n). Replaces the materialised exact lists of
+ * faster/deduplication.py:628-701 (and faster/comparison.py:500-600) inside
+ * the merge when those lists would be too large to build (the 2.6e10 pairs of
+ * a 2M-row deduplication on birth year). Only pairs listed by some listed
+ * variable are emitted; patterns whose listed digits are all 0 stay empty and
+ * their sizes follow from value-group counts on the host.
+ */
+int fh_pattern_merge_rowids(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
+                            const int32_t* var_levels, int n_exact, const int64_t* const* ids_a,
+                            const int64_t* const* ids_b, int64_t id_div, int64_t* ids, int64_t capacity,
+                            int64_t* pattern_offsets, int64_t* n_union, fh_stream_t stream);
+
+/*
  * Fellegi-Sunter EM over a pattern histogram, FP64, one workgroup running
  * every iteration on the device.
  *

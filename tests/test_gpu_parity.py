@@ -222,3 +222,43 @@ def test_pipeline_linkage_and_posteriors(oracle):
     post = comp.posteriors(est.Ksi).cpu().numpy()
     ref = np.concatenate([np.full(x.numel(), est.Ksi[k + 1]) for k, x in enumerate(comp.Indices)])
     np.testing.assert_array_equal(post, ref)
+
+
+@pytest.mark.parametrize("which", ["dedup", "link"])
+def test_rowid_exact_mode(oracle, which):
+    """Exact variables kept as row ids: Counts equal the oracle's (listed)
+    Counts, fuzzy-pattern lists are identical, and the lazy exact-only lists
+    materialise to the oracle's lists."""
+    import faster
+    from faster.patterns import RowIdPairs
+    from faster.synthetic import dedup_table, linkage_tables
+
+    vf = ["first_name", "last_name"]
+    if which == "dedup":
+        df, _ = dedup_table(900, seed=21)
+        df["birth_year"] = (df["birth_year"] % 7).astype(np.int16)
+        df["flag"] = (np.arange(len(df)) % 3).astype(np.int64)
+        obj = faster.Deduplication(df, vf, ["birth_year", "flag"])
+        obj.fit(Exact_Mode="rowid")
+        _, indices, counts = oracle.deduplication_fit([df[v].to_numpy() for v in vf],
+                                                      [df["birth_year"].to_numpy(), df["flag"].to_numpy()])
+    else:
+        dfa, dfb, _ = linkage_tables(700, 800, seed=22)
+        dfa["house_number"] = dfa["house_number"] % 5
+        dfb["house_number"] = dfb["house_number"] % 5
+        obj = faster.Comparison(dfa, dfb, vf, vf, ["house_number"], ["house_number"])
+        obj.fit(Exact_Mode="rowid")
+        _, indices, counts = oracle.comparison_fit([dfa[v].to_numpy() for v in vf], [dfb[v].to_numpy() for v in vf],
+                                                   [dfa["house_number"].to_numpy()], [dfb["house_number"].to_numpy()])
+    assert obj.Exact_Mode == "rowid"
+    np.testing.assert_array_equal(obj.Counts, counts)
+    lazy = 0
+    for k, (g, r) in enumerate(zip(obj.Indices, indices)):
+        if isinstance(g, RowIdPairs):
+            lazy += 1
+            assert g.numel() == len(r)
+            if which == "link":
+                np.testing.assert_array_equal(_np(g.materialize()), np.asarray(r, dtype=np.int64))
+        else:
+            np.testing.assert_array_equal(_np(g), np.asarray(r, dtype=np.int64), err_msg=f"pattern {k + 1}")
+    assert lazy == (3 if which == "dedup" else 1)
