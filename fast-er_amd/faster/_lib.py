@@ -71,6 +71,9 @@ _SIGS = {
                          ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp]),
     "fh_match_probability": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _vp]),
     "fh_pair_posteriors": (_i32, [_vp, _i32, _vp, _vp, _i64, _vp]),
+    "fh_fs_loglik_fwd": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _vp, _vp, _vp]),
+    "fh_fs_loglik_bwd": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _vp, _vp, _vp]),
+    "fh_fs_loglik_hess": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _vp, _vp]),
 }
 
 EXPORTED = tuple(_SIGS)

@@ -231,6 +231,28 @@ int fh_match_probability(int n_vars, const int32_t* levels, const double* params
                          fh_stream_t stream);
 
 /*
+ * Differentiable Fellegi-Sunter log-likelihood (FP64). No counterpart in the
+ * reference, which only runs EM (faster/estimation.py:98-158); these give the
+ * likelihood, its gradient and its Hessian over logits so that standard
+ * errors and gradient-based fits are available.
+ *
+ * theta (device double[1 + 2 * sum(levels)]): [a, beta, alpha] with
+ * lambda = sigmoid(a), pi_0[k] = softmax(beta_k), pi_1[k] = softmax(alpha_k)
+ * (the layout [lambda, pi_0, pi_1] of fh_em_fit's params). counts: device int64[prod(levels)].
+ *   fh_fs_loglik_fwd : *ll (device) = sum_p N_p log M_p; post (device,
+ *                      optional) = posterior match probability per pattern.
+ *   fh_fs_loglik_bwd : grad_theta = grad_out[0] * dLL/dtheta (hand-written).
+ *   fh_fs_loglik_hess: hess (device double[dim * dim], dim = len(theta)) =
+ *                      d2LL/dtheta2 in closed form.
+ */
+int fh_fs_loglik_fwd(int n_vars, const int32_t* levels, const int64_t* counts, const double* theta,
+                     double* ll, double* post, fh_stream_t stream);
+int fh_fs_loglik_bwd(int n_vars, const int32_t* levels, const int64_t* counts, const double* theta,
+                     const double* grad_out, double* grad_theta, fh_stream_t stream);
+int fh_fs_loglik_hess(int n_vars, const int32_t* levels, const int64_t* counts, const double* theta,
+                      double* hess, fh_stream_t stream);
+
+/*
  * Per-pair posterior gather: post[i] = ksi[k] for every pair i of pattern k,
  * following the pattern-major layout of fh_pattern_merge
  * (pattern_offsets device int64[n_patterns + 1]). The per-pair form of the
