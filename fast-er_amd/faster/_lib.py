@@ -76,6 +76,31 @@ _SIGS = {
     "fh_fs_loglik_hess": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _vp, _vp]),
 }
 
+
+
+class BlockedTuning(ctypes.Structure):
+    """Mirror of fh_blocked_tuning."""
+
+    _fields_ = [
+        ("tile_width", ctypes.c_int32),
+        ("tiles_per_item", ctypes.c_int32),
+        ("waves_per_item", ctypes.c_int32),
+        ("wgs_per_cu", ctypes.c_int32),
+        ("hash_slots", ctypes.c_int64),
+    ]
+
+
+_SIGS.update({
+    "fh_blocked_plan_create": (_i32, [_i32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _i64, ctypes.POINTER(_vp),
+                                      ctypes.POINTER(_vp), _i64, _vp, _vp, ctypes.c_int32, _i32,
+                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp), _f32, _f32, _f32,
+                                      ctypes.POINTER(BlockedTuning), ctypes.POINTER(_vp), _vp]),
+    "fh_blocked_run": (_i32, [_vp, _i32, _vp]),
+    "fh_blocked_counts": (_i32, [_vp, _vp, ctypes.POINTER(_i64), ctypes.POINTER(_i32), ctypes.POINTER(_f64)]),
+    "fh_blocked_info": (_i32, [_vp, ctypes.POINTER(_f64)]),
+    "fh_blocked_plan_destroy": (_i32, [_vp]),
+})
+
 EXPORTED = tuple(_SIGS)
 
 

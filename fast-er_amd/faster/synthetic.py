@@ -111,3 +111,40 @@ def dedup_table(n, seed=0, dup_frac=0.2, min_edits=1, max_edits=3, p_geom=0.5):
         cid += 1
         i += size
     return pd.DataFrame(cols), cluster
+
+
+def zipf_block_sizes(rng, n_rows, n_blocks, s=1.2, kmin=10, kmax=60000):
+    """Block sizes drawn from a continuous power law k^-s on [kmin, kmax]
+    (inverse CDF), rounded and rescaled so that they sum to ``n_rows``."""
+    a = 1.0 - s
+    u = rng.random(n_blocks)
+    k = (kmin ** a + u * (kmax ** a - kmin ** a)) ** (1.0 / a)
+    sizes = np.maximum(1, np.round(k * n_rows / k.sum())).astype(np.int64)
+    sizes[np.argmax(sizes)] += n_rows - int(sizes.sum())
+    return sizes
+
+
+def blocked_tables(n_rows, n_blocks=4000, seed=0, s=1.2, kmin=10, kmax=60000, dup_frac=0.3, max_edits=2,
+                   mean_len=8.0, sigma=0.45, max_len=32):
+    """Blocked linkage tables: rows of both sides sorted by block, Zipf block
+    sizes (same size on both sides), string lengths lognormal clipped to
+    [2, max_len] (spanning the 2-8 / 9-16 / 17-32 buckets), and ``dup_frac`` of
+    the B rows perturbed copies of A rows of the same block. Returns (df_A,
+    df_B, block ids of A rows, block ids of B rows, sizes)."""
+    rng = np.random.default_rng(seed)
+    sizes = zipf_block_sizes(rng, n_rows, n_blocks, s, kmin, kmax)
+    blk = np.repeat(np.arange(n_blocks), sizes).astype(np.int32)
+    starts = np.concatenate([[0], np.cumsum(sizes)])
+    A = {v: random_strings(rng, n_rows, mean_len, sigma, 2, max_len) for v in STRING_VARS}
+    B = {v: random_strings(rng, n_rows, mean_len, sigma, 2, max_len) for v in STRING_VARS}
+    A["house_number"] = rng.integers(1, 10000, n_rows).astype(np.int32)
+    B["house_number"] = rng.integers(1, 10000, n_rows).astype(np.int32)
+    dup = np.nonzero(rng.random(n_rows) < dup_frac)[0]
+    src = starts[blk[dup]] + (rng.random(len(dup)) * sizes[blk[dup]]).astype(np.int64)
+    for v in STRING_VARS:
+        copies = perturb_all(rng, [A[v][i] for i in src.tolist()], 0, max_edits)
+        col = B[v]
+        for d, c in zip(dup.tolist(), copies):
+            col[d] = c[:max_len] if len(c) > max_len else c
+    B["house_number"][dup] = A["house_number"][src]
+    return pd.DataFrame(A), pd.DataFrame(B), blk, blk.copy(), sizes

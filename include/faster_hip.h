@@ -91,6 +91,54 @@ int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, int64_t n_a,
                  int64_t* n_hits, fh_jw_stats* stats, fh_stream_t stream);
 
 /*
+ * Batched blocked linkage (one persistent, graph-captured launch).
+ *
+ * Replaces the reference's blocking recipe, one Comparison.fit per block with
+ * the Counts summed (docs/source/usage.rst:34-76), by a single plan over all
+ * blocks. Rows of both sides carry a block id (device int32 block_a[n_a],
+ * block_b[n_b], values in [0, n_blocks)); pairs are compared only within a
+ * block. Fuzzy variable v has device byte strings chars_a[v]/offsets_a[v]
+ * (n_a strings) and chars_b[v]/offsets_b[v] (n_b); exact variable q has
+ * device int64 value ids ids_a[q]/ids_b[q] (equal ids = equal values). The
+ * host arrays of device pointers are read during the call only.
+ *
+ * fh_blocked_plan_create sorts rows by (block, length), builds waves, tiles
+ * and largest-first work items, and allocates every buffer (it synchronises).
+ * fh_blocked_run enqueues one run on the plan's stream, ordered after
+ * `stream` and before later work on `stream`; with use_graph the run is
+ * captured into a hipGraph on first use and replayed afterwards.
+ * fh_blocked_counts waits for the last run and copies out the per-block
+ * histogram hist[block * n_patterns + code] (host int64) of every pair with at
+ * least one fuzzy level > 0 (pattern code over the fuzzy then the exact
+ * variables, first variable slowest), the number of such pairs, an overflow
+ * flag (the pair table was too small: rerun with more hash_slots) and the
+ * device time of the run in ms. Patterns whose fuzzy digits are all 0 are not
+ * counted here (their sizes follow from value-group counts per block).
+ * Strings must be 1..64 bytes (empty strings are allowed when a score of 0
+ * is below both thresholds).
+ */
+typedef struct {
+  int32_t tile_width;     /* string-2 tile width in dwords: 32, 64, 128 or 256 */
+  int32_t tiles_per_item; /* tiles per work item */
+  int32_t waves_per_item; /* string-1 waves per work item (1..8) */
+  int32_t wgs_per_cu;     /* persistent workgroups per compute unit */
+  int64_t hash_slots;     /* pair-table slots (0: 2 * (n_a + n_b), at least 2^22) */
+} fh_blocked_tuning;
+
+int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a, const int64_t* const* offsets_a,
+                           int64_t n_a, const uint8_t* const* chars_b, const int64_t* const* offsets_b,
+                           int64_t n_b, const int32_t* block_a, const int32_t* block_b, int32_t n_blocks,
+                           int n_exact, const int64_t* const* ids_a, const int64_t* const* ids_b, float p,
+                           float lower_thr, float upper_thr, const fh_blocked_tuning* tuning,
+                           void** plan, fh_stream_t stream);
+int fh_blocked_run(void* plan, int use_graph, fh_stream_t stream);
+int fh_blocked_counts(void* plan, int64_t* hist, int64_t* n_union, int* overflow, double* ms);
+/* info (host double[8]): work items (short, long strings), patterns, table
+ * slots, pairs evaluated, algorithmic lane-ops, grid size, LDS bytes. */
+int fh_blocked_info(void* plan, double* info);
+int fh_blocked_plan_destroy(void* plan);
+
+/*
  * Unique-value pair keys -> original-row pair keys (linkage).
  *
  * Replaces faster/comparison.py:429-453 (count gather, cumsum) and the

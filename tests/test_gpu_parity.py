@@ -262,3 +262,33 @@ def test_rowid_exact_mode(oracle, which):
         else:
             np.testing.assert_array_equal(_np(g), np.asarray(r, dtype=np.int64), err_msg=f"pattern {k + 1}")
     assert lazy == (3 if which == "dedup" else 1)
+
+
+def test_blocked_linkage_matches_per_block_loop(oracle):
+    """One persistent graph-captured launch over all blocks gives the per-block
+    Counts of the per-block recipe, bit for bit, on every graph replay (with
+    eager work between replays) and without the graph."""
+    from faster.blocking import BlockedLinkage, per_block_loop
+    from faster.synthetic import blocked_tables
+
+    dfa, dfb, ba, bb, sizes = blocked_tables(9000, n_blocks=40, seed=3, kmin=5, kmax=3000)
+    vf = ["first_name", "last_name", "street"]
+    bl = BlockedLinkage(dfa, dfb, ba, bb, vf, vf, ["house_number"], ["house_number"])
+    ref, _ = per_block_loop(dfa, dfb, ba, bb, vf, ["house_number"])
+    for k in range(3):
+        bl.fit(use_graph=True)
+        np.testing.assert_array_equal(bl.Block_Counts, ref, err_msg=f"replay {k}")
+        _ = torch.arange(1000, device="cuda").sum().item()  # eager work between replays
+    bl.fit(use_graph=False)
+    np.testing.assert_array_equal(bl.Block_Counts, ref)
+    # The largest block against the oracle directly.
+    b = int(np.argmax(sizes))
+    ra, rb = np.nonzero(ba == b)[0], np.nonzero(bb == b)[0]
+    _, _, counts = oracle.comparison_fit([dfa[v].to_numpy()[ra] for v in vf], [dfb[v].to_numpy()[rb] for v in vf],
+                                         [dfa["house_number"].to_numpy()[ra]], [dfb["house_number"].to_numpy()[rb]])
+    np.testing.assert_array_equal(bl.Block_Counts[b], counts)
+    # Autotuned configuration gives the same counts.
+    best, res = bl.autotune(space=dict(tile_width=[128, 256], tiles_per_item=[1, 4], wgs_per_cu=[1, 2]))
+    assert res
+    bl.fit()
+    np.testing.assert_array_equal(bl.Block_Counts, ref)
