@@ -294,7 +294,8 @@ def bench_blocked(args):
               "union_pairs": int(bl.n_union), "counts_head": bl.Counts[:6].tolist()}
     if not args.no_loops:
         log("per-block loop, reference recipe (one Comparison.fit per block)")
-        ref, t_ref = per_block_loop(dfa, dfb, ba, bb, vf, ["house_number"])
+        ref, t_ref = per_block_loop(dfa, dfb, ba, bb, vf, ["house_number"],
+                                    progress=lambda k: log(f"  recipe loop: {k} blocks done"))
         extras["recipe_loop_s"] = t_ref
         extras["recipe_counts_equal"] = bool(np.array_equal(ref, batched))
         extras["speedup_vs_recipe_loop"] = t_ref / (ms / 1e3)
@@ -310,6 +311,8 @@ def bench_blocked(args):
                  for v in vf]
             ex = group_exact_link(dfa["house_number"].to_numpy()[r], dfb["house_number"].to_numpy()[r])
             enc.append((len(r), g, ex, ex.to_device()))
+            if (b + 1) % 500 == 0:
+                log(f"  lean loop: encoded {b + 1} blocks")
         torch.cuda.synchronize()
         lean = np.zeros_like(batched)
         t0 = time.perf_counter()

@@ -253,7 +253,7 @@ class BlockedLinkage:
         return best, results
 
 
-def per_block_loop(df_A, df_B, block_A, block_B, Vars_Fuzzy, Vars_Exact, n_blocks=None, **fit_kw):
+def per_block_loop(df_A, df_B, block_A, block_B, Vars_Fuzzy, Vars_Exact, n_blocks=None, progress=None, **fit_kw):
     """The reference's blocking recipe (docs/source/usage.rst:41-76) on this
     build's plain classes: one Comparison.fit per block. Returns the per-block
     Counts (n_blocks x n_patterns) and the wall time in seconds."""
@@ -278,5 +278,7 @@ def per_block_loop(df_A, df_B, block_A, block_B, Vars_Fuzzy, Vars_Exact, n_block
         comp = Comparison(A, B, Vars_Fuzzy, Vars_Fuzzy, Vars_Exact, Vars_Exact)
         comp.fit(Exact_Mode="list", **fit_kw)
         out[b] = comp.Counts
+        if progress is not None and (b + 1) % 500 == 0:
+            progress(b + 1)
     torch.cuda.synchronize()
     return out, time.perf_counter() - t0
