@@ -42,6 +42,8 @@ def main():
                          "blocked: 8M x 8M in Zipf blocks, batched launch vs per-block loops")
     ap.add_argument("--blocks", type=int, default=4000, help="blocks of the blocked workload")
     ap.add_argument("--no-loops", action="store_true", help="blocked: skip the per-block loop baselines")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain device copy of 256 MiB three times (calibrates PMC byte counters)")
     ap.add_argument("--n", type=int, default=None, help="records per side (default 1M link, 2M dedup)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -64,6 +66,14 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_probe:
+        src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).random_(0, 255)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_per_launch": 256 << 20, "launches": 3}), flush=True)
+        return
     if args.workload == "dedup":
         return bench_dedup(args)
     if args.workload == "blocked":
