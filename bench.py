@@ -15,6 +15,7 @@ CPU baseline (the oracle's C restatement on a bounded row sample).
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
 """
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -30,6 +31,22 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def measured_traffic(workload, n):
+    """HBM bytes moved by the comparison-kernel dispatches of one step, from
+    the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
+    profiles/traffic.json (each counter scaled by what it reads for a plain
+    copy of known size). None when no record matches this workload size."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+    if not rec or int(rec.get("n", -1)) != int(n):
+        return None
+    return int(rec["bytes"])
 
 
 def main():
@@ -135,7 +152,8 @@ def main():
     t0 = time.perf_counter()
     np.random.seed(0)
     est = faster.Estimation(3, 1, counts)
-    est.fit(Tolerance=1e-6)
+    with contextlib.redirect_stdout(sys.stderr):  # stdout carries only the JSON line
+        est.fit(Tolerance=1e-6)
     t_em = time.perf_counter() - t0
 
     cpu = None
@@ -159,7 +177,8 @@ def main():
                                "thresholds [0.88, 0.94], p = 0.1", "n_a": n, "n_b": n,
                    "string_vars": list(STRING_VARS), "exact_vars": ["house_number"], "patterns": 54},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
-                     "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK, "traffic": None,
+                     "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK,
+                     "traffic": measured_traffic("link", n),
                      "kernel": "k_jw_tiles (all-pairs JW, 3 vars)", "kernel_ms": kern_ms,
                      "alg_lane_ops": alg_ops},
         "cpu_baseline": cpu,
@@ -236,7 +255,8 @@ def bench_dedup(args):
     achieved = alg_ops / (kern_ms / 1e3)
     np.random.seed(0)
     est = faster.Estimation(3, 1, counts)
-    est.fit(Tolerance=1e-6)
+    with contextlib.redirect_stdout(sys.stderr):  # stdout carries only the JSON line
+        est.fit(Tolerance=1e-6)
     line = {
         "metric": "JW pair-comparisons/s, 2M-row deduplication (upper triangle), 3 string vars",
         "value": comps / (ms / 1e3), "unit": "comparisons/s", "n_gpus": 1, "steps": args.steps,
@@ -245,8 +265,8 @@ def bench_dedup(args):
         "config": {"workload": f"deduplication of {n} records, 3 string vars + int16 birth year (row ids)",
                    "n": n, "patterns": 54},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
-                     "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK, "traffic": None,
-                     "kernel": "k_jw_tiles (triangle, 3 vars)", "kernel_ms": kern_ms, "alg_lane_ops": alg_ops},
+                     "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK,
+                     "traffic": measured_traffic("dedup", n), "kernel": "k_jw_tiles (triangle, 3 vars)", "kernel_ms": kern_ms, "alg_lane_ops": alg_ops},
         "cpu_baseline": None,
         "extras": {"level_list_sizes": [int(k.numel()) for k in keys], "counts_head": counts[:4].tolist(),
                    "exact_equal_pairs": int(patterns.exact_pattern_totals([by.group], [by.group], True)[1]),
@@ -348,8 +368,8 @@ def bench_blocked(args):
         "config": {"workload": f"blocked linkage {n} x {n} records, {args.blocks} blocks, 3 string vars + "
                                "house number", "n": n, "blocks": args.blocks},
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
-                     "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK, "traffic": None,
-                     "kernel": "k_jw_blocked (persistent, all blocks)"},
+                     "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK,
+                     "traffic": measured_traffic("blocked", n), "kernel": "k_jw_blocked (persistent, all blocks)"},
         "cpu_baseline": None,
         "extras": dict(extras, comparisons_per_s=comps / (ms / 1e3)),
     }

@@ -328,16 +328,33 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
     const int mm = c.mm;
     const int ncols = (c.cntB + 1) >> 1;
     for (int col = 0; col < ncols; col += kU) {
-      uint32_t na[kU];
+      uint32_t na[kU], cur[kU];
 #pragma unroll
-      for (int u = 0; u < kU; ++u) na[u] = init;
+      for (int u = 0; u < kU; ++u) {
+        na[u] = init;
+        cur[u] = row_ptr(c.lds, off[0], col)[u];
+      }
 #pragma unroll
       for (int i = 0; i < LA; ++i) {
-        const uint32_t* row = row_ptr(c.lds, off[i], col);
+        // Rows of character i + 1 are read while character i is processed
+        // (the scheduling fences keep the reads ahead of this step's ops).
+        uint32_t nxt[kU];
+        if (i + 1 < LA) {
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-          const uint32_t nc = or3(row[u], nwin[i], na[u]);
-          na[u] = select_bits(nc, na[u], add1_u16x2(nc));
+          for (int u = 0; u < kU; ++u) nxt[u] = row_ptr(c.lds, off[i + 1], col)[u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        uint32_t nc[kU], t[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) nc[u] = or3(cur[u], nwin[i], na[u]);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) t[u] = add1_u16x2(nc[u]);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) na[u] = select_bits(nc[u], na[u], t[u]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (i + 1 < LA) {
+#pragma unroll
+          for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
         }
       }
       // Largest match count over the 2 * kU pairs: popcount of each half.
@@ -555,13 +572,15 @@ __device__ __forceinline__ void run_item(const JwArgs& a, uint32_t* lds, int64_t
 }
 
 // LMIN..LMAX: the string-1 lengths this instance handles.
+// `ap` points to the arguments in device memory, so the rarely called
+// exact-scoring functions take them by reference without a private copy.
 template <int LMIN, int LMAX, int KIND>
-__global__ void __launch_bounds__(kThreads) k_jw_tiles(JwArgs a, int64_t wave0, int64_t nwaves, int64_t tile0,
-                                                      int64_t ntiles, int tiles_per_chunk) {
+__global__ void __launch_bounds__(kThreads) k_jw_tiles(const JwArgs* __restrict__ ap, int64_t wave0, int64_t nwaves,
+                                                      int64_t tile0, int64_t ntiles, int tiles_per_chunk) {
   extern __shared__ uint32_t lds[];
   const int64_t t_begin = tile0 + (int64_t)blockIdx.y * tiles_per_chunk;
   const int64_t t_end = min(t_begin + tiles_per_chunk, tile0 + ntiles);
-  run_item<LMIN, LMAX, KIND>(a, lds, wave0 + (int64_t)blockIdx.x * kWaves, wave0 + nwaves, t_begin, t_end);
+  run_item<LMIN, LMAX, KIND>(*ap, lds, wave0 + (int64_t)blockIdx.x * kWaves, wave0 + nwaves, t_begin, t_end);
 }
 
 // Pairs with a special string: rows = special string-1 entries (sorted
@@ -731,7 +750,7 @@ struct Plan {
 static int lrange_of(int len) { return len <= 16 ? 0 : (len <= 32 ? 1 : 2); }
 
 template <int LMIN, int LMAX, int KIND>
-static int launch_tiles(const JwArgs& args, size_t lds_bytes, int64_t w0, int64_t nw, int64_t t0, int64_t nt,
+static int launch_tiles(const JwArgs* args, size_t lds_bytes, int64_t w0, int64_t nw, int64_t t0, int64_t nt,
                         hipStream_t s) {
   if (nw <= 0 || nt <= 0) return FH_OK;
   const int64_t gx = ceil_div(nw, kWaves);
@@ -749,7 +768,7 @@ static int launch_tiles(const JwArgs& args, size_t lds_bytes, int64_t w0, int64_
 }
 
 template <int LMIN, int LMAX>
-static int launch_kinds(const JwArgs& args, size_t lds, const Plan& pl, int r, hipStream_t s, int* launches) {
+static int launch_kinds(const JwArgs* args, size_t lds, const Plan& pl, int r, hipStream_t s, int* launches) {
   const int64_t w0 = pl.wave_lo[r], nw = pl.wave_hi[r] - pl.wave_lo[r];
   if (nw <= 0) return FH_OK;
   for (int k = 0; k < 3; ++k) {
@@ -1001,6 +1020,9 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   args.code_stride = 0;
   args.overflow = nullptr;
 
+  JwArgs* d_args = nullptr;
+  FH_TRY(scr.alloc(&d_args, 1));
+  FH_HIP(hipMemcpyAsync(d_args, &args, sizeof(JwArgs), hipMemcpyHostToDevice, s));
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   if (stats) {
     FH_HIP(hipEventCreate(&ev0));
@@ -1008,9 +1030,9 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     FH_HIP(hipEventRecord(ev0, s));
   }
   int launches = 0;
-  FH_TRY((launch_kinds<1, 16>(args, lds_bytes, pl, 0, s, &launches)));
-  FH_TRY((launch_kinds<17, 32>(args, lds_bytes, pl, 1, s, &launches)));
-  FH_TRY((launch_kinds<33, 64>(args, lds_bytes, pl, 2, s, &launches)));
+  FH_TRY((launch_kinds<1, 16>(d_args, lds_bytes, pl, 0, s, &launches)));
+  FH_TRY((launch_kinds<17, 32>(d_args, lds_bytes, pl, 1, s, &launches)));
+  FH_TRY((launch_kinds<33, 64>(d_args, lds_bytes, pl, 2, s, &launches)));
 
   // Generic pairs: special strings (longer than 64 bytes, and empty strings
   // when a score of 0 reaches level 1).
