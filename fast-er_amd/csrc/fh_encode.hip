@@ -1,0 +1,197 @@
+// Device grouping of a string column by value (unique values + their rows).
+//
+// The reference takes the unique values of every column on the host with
+// np.unique over Python strings (faster/comparison.py:377,395,
+// faster/deduplication.py:420) and orders the rows of each value with a
+// stable argsort of the inverse (faster/comparison.py:380-391). Here the rows
+// are sorted on the device by their bytes: a stable LSD radix sort over the
+// big-endian 8-byte words of the zero-padded strings, least significant word
+// first, after a first pass on the length (so a string sorts before its
+// extensions by NUL bytes). The order is byte-lexicographic with shorter
+// prefixes first, which for UTF-8 is the code-point order np.unique gives,
+// and rows of one value stay ascending because every pass is stable and the
+// sort starts from the identity permutation.
+#include <algorithm>
+
+#include "fh_common.h"
+
+namespace fh {
+namespace {
+
+constexpr int kEncThreads = 256;
+
+__global__ void __launch_bounds__(kEncThreads) k_enc_init(const int64_t* __restrict__ off, int64_t n,
+                                                          uint64_t* __restrict__ perm, uint64_t* __restrict__ lens,
+                                                          unsigned long long* __restrict__ max_len) {
+  unsigned long long m = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t len = (uint64_t)(off[i + 1] - off[i]);
+    perm[i] = (uint64_t)i;
+    lens[i] = len;
+    m = m > len ? m : len;
+  }
+  // Wave maximum, then one atomic per wave.
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long o = __shfl_xor(m, d);
+    m = m > o ? m : o;
+  }
+  if (__lane_id() == 0) atomicMax(max_len, m);
+}
+
+// keys[i] = bytes [8 * word, 8 * word + 8) of row perm[i], zero-padded,
+// first byte most significant.
+__global__ void __launch_bounds__(kEncThreads) k_enc_word_keys(const uint8_t* __restrict__ chars,
+                                                               const int64_t* __restrict__ off,
+                                                               const uint64_t* __restrict__ perm, int64_t n, int word,
+                                                               uint64_t* __restrict__ keys) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = perm[i];
+    const int64_t o = off[r];
+    const int64_t len = off[r + 1] - o;
+    const int64_t b0 = (int64_t)word * 8;
+    uint64_t w = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint64_t byte = b0 + k < len ? (uint64_t)chars[o + b0 + k] : 0ull;
+      w = (w << 8) | byte;
+    }
+    keys[i] = w;
+  }
+}
+
+__device__ __forceinline__ bool same_string(const uint8_t* __restrict__ chars, const int64_t* __restrict__ off,
+                                            uint64_t x, uint64_t y) {
+  const int64_t ox = off[x], oy = off[y];
+  const int64_t len = off[x + 1] - ox;
+  if (len != off[y + 1] - oy) return false;
+  for (int64_t k = 0; k < len; ++k)
+    if (chars[ox + k] != chars[oy + k]) return false;
+  return true;
+}
+
+// heads[i] = 1 where sorted position i starts a new value; rows[i] = perm[i].
+__global__ void __launch_bounds__(kEncThreads) k_enc_heads(const uint8_t* __restrict__ chars,
+                                                           const int64_t* __restrict__ off,
+                                                           const uint64_t* __restrict__ perm, int64_t n,
+                                                           uint64_t* __restrict__ heads, int64_t* __restrict__ rows) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t r = perm[i];
+    rows[i] = (int64_t)r;
+    heads[i] = (i == 0 || !same_string(chars, off, r, perm[i - 1])) ? 1ull : 0ull;
+  }
+}
+
+// For every head: starts[g] = i and ulen[g] = its length (g = exclusive scan
+// of the heads). The closing entry starts[n_unique] = n.
+__global__ void __launch_bounds__(kEncThreads) k_enc_emit(const int64_t* __restrict__ off,
+                                                          const uint64_t* __restrict__ perm,
+                                                          const uint64_t* __restrict__ heads,
+                                                          const uint64_t* __restrict__ gid, int64_t n,
+                                                          const uint64_t* __restrict__ n_unique,
+                                                          int64_t* __restrict__ starts, uint64_t* __restrict__ ulen) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (heads[i]) {
+      const uint64_t g = gid[i];
+      const uint64_t r = perm[i];
+      starts[g] = i;
+      ulen[g] = (uint64_t)(off[r + 1] - off[r]);
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) starts[*n_unique] = n;
+}
+
+// Copies the bytes of every unique value (one thread per value).
+__global__ void __launch_bounds__(kEncThreads) k_enc_copy(const uint8_t* __restrict__ chars,
+                                                          const int64_t* __restrict__ off,
+                                                          const int64_t* __restrict__ rows,
+                                                          const int64_t* __restrict__ starts,
+                                                          const int64_t* __restrict__ uoff, int64_t n_unique,
+                                                          uint8_t* __restrict__ uchars) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_unique;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = rows[starts[g]];
+    const int64_t o = off[r];
+    const int64_t len = off[r + 1] - o;
+    uint8_t* dst = uchars + uoff[g];
+    for (int64_t k = 0; k < len; ++k) dst[k] = chars[o + k];
+  }
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kEncThreads), 8192)); }
+
+}  // namespace
+}  // namespace fh
+
+using namespace fh;
+
+extern "C" int fh_group_strings(const uint8_t* chars, const int64_t* offsets, int64_t n, uint8_t* uchars,
+                                int64_t* uoffsets, int64_t* starts, int64_t* rows, int64_t* n_unique,
+                                int64_t* unique_bytes, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n >= 0, "fh_group_strings: n < 0");
+  FH_REQUIRE(n_unique && unique_bytes, "fh_group_strings: null output count");
+  FH_REQUIRE(n < (int64_t)0xFFFFFFFF, "fh_group_strings: n = %lld exceeds 2^32 - 1", (long long)n);
+  hipStream_t s = (hipStream_t)stream;
+  if (n == 0) {
+    const int64_t zero = 0;
+    FH_HIP(hipMemcpyAsync(uoffsets, &zero, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    FH_HIP(hipMemcpyAsync(starts, &zero, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    FH_HIP(hipStreamSynchronize(s));
+    *n_unique = 0;
+    *unique_bytes = 0;
+    return FH_OK;
+  }
+  Scratch scr(s);
+  uint64_t *perm = nullptr, *perm_tmp = nullptr, *keys = nullptr, *keys_tmp = nullptr, *small = nullptr;
+  FH_TRY(scr.alloc(&perm, (size_t)n));
+  FH_TRY(scr.alloc(&perm_tmp, (size_t)n));
+  FH_TRY(scr.alloc(&keys, (size_t)n + 1));
+  FH_TRY(scr.alloc(&keys_tmp, (size_t)n + 1));
+  FH_TRY(scr.alloc(&small, 4));  // [0] max length, [1] unique count, [2] unique bytes
+  FH_TRY(fill_u64(small, 0, 4, s));
+  const unsigned grid = grid_for(n);
+
+  // Pass 0: lengths (also the maximum length, which sets the word count).
+  hipLaunchKernelGGL(k_enc_init, dim3(grid), dim3(kEncThreads), 0, s, offsets, n, perm, keys,
+                     reinterpret_cast<unsigned long long*>(small));
+  FH_HIP(hipGetLastError());
+  uint64_t max_len = 0;
+  FH_HIP(hipMemcpyAsync(&max_len, small, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, n, 0, std::max(1, bits_for(max_len)), s));
+
+  // Words from the last to the first; bytes past max_len are zero in every
+  // key, so the low bits of a partial last word need no pass.
+  const int words = (int)ceil_div((int64_t)max_len, 8);
+  for (int w = words - 1; w >= 0; --w) {
+    hipLaunchKernelGGL(k_enc_word_keys, dim3(grid), dim3(kEncThreads), 0, s, chars, offsets, perm, n, w, keys);
+    FH_HIP(hipGetLastError());
+    const int valid_bytes = (int)std::min<int64_t>(8, (int64_t)max_len - 8 * (int64_t)w);
+    FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, n, 8 * (8 - valid_bytes), 64, s));
+  }
+
+  // Value boundaries, value ids, starts and packed unique bytes.
+  uint64_t* heads = keys;
+  uint64_t* gid = keys_tmp;
+  hipLaunchKernelGGL(k_enc_heads, dim3(grid), dim3(kEncThreads), 0, s, chars, offsets, perm, n, heads, rows);
+  FH_HIP(hipGetLastError());
+  FH_TRY(exclusive_scan_u64(heads, gid, n, small + 1, s));
+  uint64_t* ulen = perm_tmp;
+  hipLaunchKernelGGL(k_enc_emit, dim3(grid), dim3(kEncThreads), 0, s, offsets, perm, heads, gid, n, small + 1,
+                     starts, ulen);
+  FH_HIP(hipGetLastError());
+  uint64_t nu = 0;
+  FH_HIP(hipMemcpyAsync(&nu, small + 1, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  FH_TRY(exclusive_scan_u64(ulen, reinterpret_cast<uint64_t*>(uoffsets), (int64_t)nu, small + 2, s));
+  FH_HIP(hipMemcpyAsync(uoffsets + nu, small + 2, sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_enc_copy, dim3(grid_for((int64_t)nu)), dim3(kEncThreads), 0, s, chars, offsets, rows, starts,
+                     uoffsets, (int64_t)nu, uchars);
+  FH_HIP(hipGetLastError());
+  uint64_t ub = 0;
+  FH_HIP(hipMemcpyAsync(&ub, small + 2, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  *n_unique = (int64_t)nu;
+  *unique_bytes = (int64_t)ub;
+  return FH_OK;
+}

@@ -58,6 +58,8 @@ _SIGS = {
     "fh_exact_link": (_i32, [_vp, _i64, _vp, _vp, _i64, _i64, _vp, _i64, ctypes.POINTER(_i64), _vp]),
     "fh_exact_dedup": (_i32, [_vp, _i64, _vp, _vp, _i64, _vp, _i64, ctypes.POINTER(_i64), _vp]),
     "fh_sort_u64": (_i32, [_vp, _i64, _i32, _i32, _vp]),
+    "fh_group_strings": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                _vp]),
     "fh_split_levels": (_i32, [_vp, _i64, _i32, _vp, ctypes.POINTER(_i64), _vp]),
     "fh_sorted_member_mask": (_i32, [_vp, _i64, _vp, _i64, _i32, _vp, _vp]),
     "fh_pattern_merge": (_i32, [_i32, ctypes.POINTER(_vp), ctypes.POINTER(_i64),

@@ -126,6 +126,24 @@ def exact_dedup(group, starts, rows, n_groups, capacity=None):
     return out[:need]
 
 
+def group_strings(chars: torch.Tensor, offsets: torch.Tensor, n: int):
+    """Device grouping of n packed strings (fh_group_strings). Returns
+    (unique chars, unique offsets, starts, rows, n_unique)."""
+    L = _lib.load()
+    dev = _lib.device()
+    total = max(int(chars.numel()), 1)
+    uchars = torch.empty(total, dtype=torch.uint8, device=dev)
+    uoffsets = _empty_i64(n + 1)
+    starts = _empty_i64(n + 1)
+    rows = _empty_i64(n)
+    nu = ctypes.c_int64(0)
+    ub = ctypes.c_int64(0)
+    check(L.fh_group_strings(ptr(chars), ptr(offsets), _i64(n), ptr(uchars), ptr(uoffsets), ptr(starts), ptr(rows),
+                             ctypes.byref(nu), ctypes.byref(ub), stream()), "fh_group_strings")
+    k = nu.value
+    return uchars[:max(ub.value, 1)], uoffsets[:k + 1], starts[:k + 1], rows[:n], k
+
+
 def sort_keys(keys: torch.Tensor, max_key: int) -> torch.Tensor:
     """In-place ascending radix sort of level keys below ``max_key``."""
     L = _lib.load()

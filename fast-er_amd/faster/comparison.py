@@ -17,7 +17,7 @@ import pandas as pd
 import torch
 
 from . import _lib, _ops, patterns
-from .encode import group_exact_link, group_strings, pack_strings
+from .encode import group_exact_link, group_strings_device, pack_strings
 from .levels import level_of_score
 
 _LEVELS_FUZZY = 3
@@ -75,8 +75,8 @@ def jaro_winkler_unique_gpu(str_A, str_B, p=0.1, lower_thr=0.88, upper_thr=0.94,
     :return: two ascending int64 tensors of ids ``i * len(str_B) + j``.
     """
     _lib.load()
-    ga = group_strings(str_A).to_device()
-    gb = group_strings(str_B).to_device()
+    ga = group_strings_device(str_A)
+    gb = group_strings_device(str_B)
     return _split(fuzzy_keys_link(ga, gb, p, lower_thr, upper_thr), 3)
 
 
@@ -146,8 +146,8 @@ class Comparison():
         stats = []
         keys = []
         for va, vb in zip(self.Vars_Fuzzy_A, self.Vars_Fuzzy_B):
-            ga = group_strings(self.df_A[va].to_numpy()).to_device()
-            gb = group_strings(self.df_B[vb].to_numpy()).to_device()
+            ga = group_strings_device(self.df_A[va].to_numpy())
+            gb = group_strings_device(self.df_B[vb].to_numpy())
             keys.append(fuzzy_keys_link(ga, gb, p, Lower_Thr, Upper_Thr, stats))
             del ga, gb
         exact = [group_exact_link(self.df_A[va].to_numpy(), self.df_B[vb].to_numpy())

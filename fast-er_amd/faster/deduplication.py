@@ -16,7 +16,7 @@ import torch
 
 from . import _lib, _ops, patterns
 from .comparison import _split
-from .encode import group_exact_dedup, group_strings
+from .encode import group_exact_dedup, group_strings_device
 from .levels import level_of_score
 
 _LEVELS_FUZZY = 3
@@ -45,7 +45,7 @@ def jaro_winkler_dedup_gpu(string, p=0.1, lower_thr=0.88, upper_thr=0.94, num_th
     :return: two ascending int64 tensors of ids ``i * len(string) + j``, i > j.
     """
     _lib.load()
-    g = group_strings(string).to_device()
+    g = group_strings_device(string)
     return _split(fuzzy_keys_dedup(g, p, lower_thr, upper_thr), 3)
 
 
@@ -97,7 +97,7 @@ class Deduplication():
         stats = []
         keys = []
         for v in self.Vars_Fuzzy:
-            g = group_strings(self.df[v].to_numpy()).to_device()
+            g = group_strings_device(self.df[v].to_numpy())
             keys.append(fuzzy_keys_dedup(g, p, Lower_Thr, Upper_Thr, stats))
             del g
         exact = [group_exact_dedup(self.df[v].to_numpy()) for v in self.Vars_Exact]

@@ -73,6 +73,47 @@ def pack_strings(values) -> tuple[np.ndarray, np.ndarray]:
     return chars, offsets
 
 
+def pack_column(values) -> tuple[np.ndarray, np.ndarray]:
+    """Packs a column's values as strings (UTF-8 bytes + int64 offsets),
+    through Arrow's buffers when every value is already a ``str``; other
+    values go through ``str()`` first, as in group_strings."""
+    arr = _to_str_array(values)
+    try:
+        import pyarrow as pa
+    except ImportError:  # pragma: no cover - pyarrow is optional
+        pa = None
+    if pa is not None:
+        try:
+            pa_arr = pa.array(arr, type=pa.large_string(), from_pandas=False)
+        except (pa.ArrowInvalid, pa.ArrowTypeError):
+            pa_arr = None
+        if pa_arr is not None and pa_arr.null_count == 0:
+            bufs = pa_arr.buffers()
+            n = len(pa_arr)
+            offs = np.frombuffer(bufs[1], dtype=np.int64, count=n + 1, offset=pa_arr.offset * 8)
+            base = int(offs[0])
+            offsets = (offs - base).astype(np.int64)
+            total = int(offsets[-1])
+            chars = (np.frombuffer(bufs[2], dtype=np.uint8, count=total, offset=base).copy()
+                     if total else np.zeros(0, np.uint8))
+            return chars, offsets
+    return pack_strings([s if isinstance(s, str) else str(s) for s in arr])
+
+
+def group_strings_device(values) -> DeviceStringGroups:
+    """Unique values of a string column with their rows (ascending), grouped
+    on the device (fh_group_strings): the host only packs the bytes."""
+    from . import _ops
+
+    chars, offsets = pack_column(values)
+    n = len(offsets) - 1
+    dev = _lib.device()
+    d_chars = torch.from_numpy(chars if chars.size else np.zeros(1, np.uint8)).to(dev)
+    d_off = torch.from_numpy(offsets).to(dev)
+    uchars, uoff, starts, rows, nu = _ops.group_strings(d_chars, d_off, n)
+    return DeviceStringGroups(n_rows=n, n_unique=nu, chars=uchars, offsets=uoff, starts=starts, rows=rows)
+
+
 def group_strings(values) -> StringGroups:
     """Unique values of a string column with their rows (ascending)."""
     arr = _to_str_array(values)

@@ -139,6 +139,23 @@ int fh_blocked_info(void* plan, double* info);
 int fh_blocked_plan_destroy(void* plan);
 
 /*
+ * Groups the n rows of a string column by value on the device.
+ *
+ * Replaces the host np.unique of faster/comparison.py:377,395 and
+ * faster/deduplication.py:420 with the stable argsort of the inverse at
+ * faster/comparison.py:380-391. chars/offsets (device, offsets int64[n + 1])
+ * hold the rows' UTF-8 bytes. Outputs (device, caller-sized): uchars
+ * (capacity offsets[n] bytes) and uoffsets (int64[n + 1]) the unique values
+ * in byte-lexicographic order (shorter prefix first: np.unique's order for
+ * str); starts (int64[n + 1]) and rows (int64[n]) list the rows of unique
+ * value u as rows[starts[u] .. starts[u + 1]), ascending. *n_unique and
+ * *unique_bytes are host outputs. Synchronizes the stream.
+ */
+int fh_group_strings(const uint8_t* chars, const int64_t* offsets, int64_t n, uint8_t* uchars,
+                     int64_t* uoffsets, int64_t* starts, int64_t* rows, int64_t* n_unique,
+                     int64_t* unique_bytes, fh_stream_t stream);
+
+/*
  * Unique-value pair keys -> original-row pair keys (linkage).
  *
  * Replaces faster/comparison.py:429-453 (count gather, cumsum) and the

@@ -181,6 +181,18 @@ def _unique_groups(values):
     return uniq, inv, cnt.astype(np.int64), rows_sorted, starts
 
 
+def group_strings(values):
+    """Unique values of a string column (np.unique over Python ``str``,
+    comparison.py:377,395 / deduplication.py:420) as UTF-8 bytes, with the
+    rows of each value in ascending order (comparison.py:380-391).
+    Returns (list of bytes, starts int64[n_unique + 1], rows int64[n])."""
+    arr = np.array([s if isinstance(s, str) else str(s) for s in values], dtype=object)
+    if len(arr) == 0:
+        return [], np.zeros(1, np.int64), np.zeros(0, np.int64)
+    uniq, _, _, rows, starts = _unique_groups(arr)
+    return [u.encode("utf-8") for u in uniq], starts, rows
+
+
 def _expand_link(ua, ub, cntA, cntB, rowsA, startsA, rowsB, startsB, n_B):
     """Unique-pair -> original-pair expansion (the kernel at
     comparison.py:196-245): each unique pair (a, b) yields

@@ -152,3 +152,21 @@ def test_bench_refuses_multi_gpu():
                        text=True, timeout=120)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_pack_column_matches_pack_strings():
+    """Arrow packing gives the same bytes and offsets as the plain packer, for
+    str columns (also a sliced pandas column) and for values that need str()."""
+    import pandas as pd
+
+    from faster.encode import pack_column, pack_strings
+
+    rng = np.random.default_rng(2)
+    words = ["".join("abcé日"[k] for k in rng.integers(0, 5, rng.integers(0, 9))) for _ in range(500)]
+    for values in (words, pd.Series(words[100:]).to_numpy(), pd.Series(words).iloc[37:301].to_numpy(),
+                   np.array(words, dtype=str), [1, "a", None, 2.5, "None"], []):
+        c1, o1 = pack_column(values)
+        c2, o2 = pack_strings([v if isinstance(v, str) else str(v) for v in values])
+        np.testing.assert_array_equal(o1, o2)
+        np.testing.assert_array_equal(c1, c2)
+        assert o1.dtype == np.int64 and c1.dtype == np.uint8

@@ -297,3 +297,22 @@ def test_linkage_pattern_zero_quirk(oracle):
     np.testing.assert_array_equal(a * 4 + b, [7, 9])
     with pytest.raises(Exception, match="No pair of observations"):
         oracle.linkage_pairs(idx, np.zeros(4), n_B=4)
+
+
+def test_group_strings_order_and_rows(oracle):
+    """Unique values in Python's str order (shorter prefix first, NUL after the
+    bare prefix, code points for non-ASCII) with ascending rows: a hand-worked
+    case and a brute-force check of every group."""
+    vals = ["b", "a", "b", "", "ab", "a\x00", "a", "é", "z"]
+    uniq, starts, rows = oracle.group_strings(vals)
+    assert uniq == [b"", b"a", b"a\x00", b"ab", b"b", b"z", "é".encode()]
+    assert starts.tolist() == [0, 1, 3, 4, 5, 7, 8, 9]
+    assert rows.tolist() == [3, 1, 6, 5, 4, 0, 2, 8, 7]
+    rng = np.random.default_rng(5)
+    pool = ["", "a", "ab", "abc", "b", "ba", "a\x00b", "zz", "ü", "日本"]
+    vals = [pool[i] for i in rng.integers(0, len(pool), 300)]
+    uniq, starts, rows = oracle.group_strings(vals)
+    assert uniq == sorted({v.encode() for v in vals})  # UTF-8 byte order = code-point order
+    for u, s0, s1 in zip(uniq, starts[:-1], starts[1:]):
+        assert rows[s0:s1].tolist() == [i for i, v in enumerate(vals) if v.encode() == u]
+    assert oracle.group_strings([])[0] == []
