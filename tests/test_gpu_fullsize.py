@@ -57,8 +57,8 @@ def test_sampled_rows_match_oracle(oracle, fullsize):
 
 def test_invariants(fullsize):
     dfa, dfb, keys, ids, offs, n_union = fullsize
-    for k in keys:
-        idv = k >> 2
+    idvs = [k >> 2 for k in keys]
+    for k, idv in zip(keys, idvs):
         assert np.all(np.diff(idv) > 0)
         assert idv.min() >= 0 and idv.max() < N * N
         assert set(np.unique(k & 3).tolist()) <= {1, 2}
@@ -69,21 +69,19 @@ def test_invariants(fullsize):
     assert len(keys[3]) == int(np.dot(ca, cb))
     counts = np.concatenate([[N * N - n_union], np.diff(np.asarray(offs))[1:]])
     assert counts.sum() == N * N
-    assert n_union == len(np.unique(np.concatenate([k >> 2 for k in keys])))
+    assert n_union == len(np.unique(np.concatenate(idvs)))
     # Re-derive the pattern code of sampled union pairs from the per-variable lists.
     rng = np.random.default_rng(7)
-    pick = rng.choice(len(ids), 2000, replace=False)
+    pick = np.sort(rng.choice(len(ids), 2000, replace=False))
+    pid = ids[pick]
     strides = [18, 6, 2, 1]
-    bounds = np.asarray(offs)
-    for p in pick.tolist():
-        pid = ids[p]
-        code = 0
-        for v, k in enumerate(keys):
-            j = np.searchsorted(k >> 2, pid)
-            if j < len(k) and (k[j] >> 2) == pid:
-                code += int(k[j] & 3) * strides[v]
-        owner = int(np.searchsorted(bounds, p, side="right") - 1)
-        assert owner == code
+    code = np.zeros(len(pick), dtype=np.int64)
+    for v, (k, idv) in enumerate(zip(keys, idvs)):
+        j = np.minimum(np.searchsorted(idv, pid), len(k) - 1)
+        hit = idv[j] == pid
+        code += np.where(hit, (k[j] & 3) * strides[v], 0)
+    owner = np.searchsorted(np.asarray(offs), pick, side="right") - 1
+    np.testing.assert_array_equal(owner, code)
     # pattern lists ascending
     for c in range(1, len(offs) - 1):
         seg = ids[offs[c]:offs[c + 1]]
