@@ -1,0 +1,570 @@
+// Batched blocked linkage (fh_blocked_*): the comparison of every block in
+// one persistent, graph-captured launch. The comparison kernel is in
+// fh_jw_core.h.
+#include <memory>
+
+#include "fh_jw_core.h"
+
+// ===========================================================================
+// Batched blocked linkage: every (variable, block, wave group, tile chunk)
+// work item of every block in one persistent launch.
+//
+// The reference links blocks by running one Comparison per block and summing
+// the Counts (docs/source/usage.rst:34-76). Here rows carry a block id, the
+// plan sorts each side by (block, length), builds waves and tiles inside
+// blocks, and lists work items largest first; a persistent kernel pops them
+// from an atomic queue. Hits go straight into an open-addressing table keyed
+// by pair id whose value accumulates the pattern code over the variables; a
+// second kernel adds the exact-variable digits (row-id lookups) and counts
+// (block, pattern). The sequence is captured once into a hipGraph and
+// replayed.
+// ===========================================================================
+namespace fh {
+namespace {
+
+constexpr int kMaxBlkVars = 8;
+constexpr int kMaxBlkExact = 8;
+
+struct WorkItem {
+  int32_t var;
+  int32_t nwaves;
+  int64_t wave0;
+  int64_t tile0;
+  int64_t ntiles;
+  double cost;
+};
+
+template <int LMIN, int LMAX>
+__global__ void __launch_bounds__(kThreads) k_jw_blocked(const JwArgs* __restrict__ vars,
+                                                        const WorkItem* __restrict__ items, int64_t n_items,
+                                                        unsigned int* head) {
+  extern __shared__ uint32_t lds[];
+  __shared__ int64_t cur;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) cur = (int64_t)atomicAdd(head, 1u);
+    __syncthreads();
+    const int64_t it = cur;
+    if (it >= n_items) break;  // same value in every wave: the whole workgroup exits
+    const WorkItem w = items[it];
+    run_item<LMIN, LMAX, kMixed>(vars[w.var], lds, w.wave0, w.wave0 + w.nwaves, w.tile0, w.tile0 + w.ntiles);
+  }
+}
+
+struct ExactRowIds {
+  const int64_t* a[kMaxBlkExact];
+  const int64_t* b[kMaxBlkExact];
+  uint32_t stride[kMaxBlkExact];
+  int n;
+};
+
+__global__ void __launch_bounds__(256) k_blocked_hist(const unsigned long long* __restrict__ hkeys,
+                                                      const uint32_t* __restrict__ hcodes, uint64_t nslots,
+                                                      uint64_t n_b, const int32_t* __restrict__ block_a,
+                                                      ExactRowIds ex, uint64_t n_patterns,
+                                                      unsigned long long* __restrict__ hist,
+                                                      unsigned long long* __restrict__ n_union) {
+  unsigned long long mine = 0;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long key = hkeys[s];
+    if (key == kEmptyKey) continue;
+    const uint64_t i = key / n_b, j = key % n_b;
+    uint32_t code = hcodes[s];
+    for (int q = 0; q < ex.n; ++q)
+      if (ex.a[q][i] == ex.b[q][j]) code += ex.stride[q];
+    atomicAdd(hist + (uint64_t)block_a[i] * n_patterns + code, 1ull);
+    ++mine;
+  }
+  for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o, 64);
+  if ((threadIdx.x & 63) == 0 && mine) atomicAdd(n_union, mine);
+}
+
+// Sort keys for one side: block * kNumClasses + length class; presence bits.
+__global__ void __launch_bounds__(256) k_blk_classify(const uint8_t* __restrict__ chars, const int64_t* __restrict__ off,
+                                                      int64_t n, const int32_t* __restrict__ block,
+                                                      uint32_t* __restrict__ key, uint64_t* __restrict__ ent,
+                                                      uint32_t* __restrict__ hist, uint32_t* __restrict__ present,
+                                                      uint32_t* __restrict__ maxlen) {
+  __shared__ uint32_t pr[8];
+  if (threadIdx.x < 8) pr[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = off[e];
+    const int64_t len = off[e + 1] - o;
+    const uint32_t c = len > kLongLen ? (uint32_t)kClassLong : (uint32_t)len;
+    const uint32_t k = (uint32_t)block[e] * kNumClasses + c;
+    key[e] = k;
+    ent[e] = (uint64_t)e;
+    atomicAdd(hist + k, 1u);
+    if (len > kLongLen) atomicMax(maxlen, len > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)len);
+    if (len >= 1 && len <= kLongLen)
+      for (int64_t i = 0; i < len; ++i) {
+        const uint32_t b = chars[o + i];
+        atomicOr(&pr[b >> 5], 1u << (b & 31));
+      }
+  }
+  __syncthreads();
+  if (threadIdx.x < 8 && pr[threadIdx.x]) atomicOr(&present[threadIdx.x], pr[threadIdx.x]);
+}
+
+struct BlockedPlan {
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int n_vars = 0;
+  int64_t n_a = 0, n_b = 0;
+  int32_t n_blocks = 0;
+  uint64_t n_patterns = 0;
+  int tw = 256;
+  size_t lds_bytes = 0;
+  int grid = 0;
+  // per variable
+  std::vector<void*> bufs;  // every device allocation of the plan
+  std::vector<JwArgs> args;
+  std::vector<const uint8_t*> chars_b;
+  std::vector<const int64_t*> off_b;
+  std::vector<uint64_t*> sorted_b;
+  std::vector<TileMeta*> tiles_dev;
+  std::vector<int64_t> ntiles;
+  std::vector<uint8_t*> amap_dev;
+  std::vector<int> krows;
+  std::vector<uint32_t*> table;
+  JwArgs* args_dev = nullptr;
+  WorkItem* items_dev[2] = {nullptr, nullptr};
+  int64_t n_items[2] = {0, 0};
+  unsigned int* heads = nullptr;
+  unsigned long long* hkeys = nullptr;
+  uint32_t* hcodes = nullptr;
+  uint64_t hslots = 0;
+  unsigned int* overflow = nullptr;
+  unsigned long long* hist = nullptr;
+  unsigned long long* n_union = nullptr;
+  const int32_t* block_a = nullptr;
+  ExactRowIds ex{};
+  double pairs_kernel = 0, alg_ops = 0;
+
+  template <typename T>
+  int alloc(T** out, size_t count) {
+    size_t bytes = std::max<size_t>(256, count * sizeof(T));
+    void* ptr = nullptr;
+    hipError_t e = hipMalloc(&ptr, bytes);
+    if (e != hipSuccess) {
+      set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+      return FH_ERR_HIP;
+    }
+    bufs.push_back(ptr);
+    *out = static_cast<T*>(ptr);
+    return FH_OK;
+  }
+
+  ~BlockedPlan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* ptr : bufs) (void)hipFree(ptr);
+    bufs.clear();
+    for (hipEvent_t e : {ev_in, ev_out, ev_t0, ev_t1})
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // The device work of one run, enqueued on `stream` (capturable: fill
+  // kernels only, no allocation, no host synchronisation).
+  int enqueue() {
+    FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(hkeys), kEmptyKey, (int64_t)hslots, stream));
+    FH_TRY(fill_u32(hcodes, 0, (int64_t)hslots, stream));
+    FH_TRY(fill_u32(heads, 0, 2, stream));
+    FH_TRY(fill_u32(overflow, 0, 1, stream));
+    FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(hist), 0, (int64_t)(n_patterns * (uint64_t)n_blocks), stream));
+    FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(n_union), 0, 1, stream));
+    for (int v = 0; v < n_vars; ++v) {
+      if (ntiles[v] == 0) continue;
+      const int64_t total = ntiles[v] * tw;
+      const int64_t bx = std::min<int64_t>(ceil_div(total, 256), 65535);
+      hipLaunchKernelGGL(k_build_table, dim3((unsigned)bx, (unsigned)krows[v]), dim3(256), 0, stream, chars_b[v],
+                         off_b[v], sorted_b[v], tiles_dev[v], ntiles[v], amap_dev[v], krows[v], tw, table[v]);
+      FH_HIP(hipGetLastError());
+    }
+    if (n_items[0] > 0) {
+      hipLaunchKernelGGL((k_jw_blocked<1, 32>), dim3((unsigned)grid), dim3(kThreads), lds_bytes, stream, args_dev,
+                         items_dev[0], n_items[0], heads);
+      FH_HIP(hipGetLastError());
+    }
+    if (n_items[1] > 0) {
+      hipLaunchKernelGGL((k_jw_blocked<33, 64>), dim3((unsigned)grid), dim3(kThreads), lds_bytes, stream, args_dev,
+                         items_dev[1], n_items[1], heads + 1);
+      FH_HIP(hipGetLastError());
+    }
+    const int64_t hb = std::min<int64_t>(ceil_div((int64_t)hslots, 256), 16384);
+    hipLaunchKernelGGL(k_blocked_hist, dim3((unsigned)hb), dim3(256), 0, stream, hkeys, hcodes, hslots, (uint64_t)n_b,
+                       block_a, ex, n_patterns, hist, n_union);
+    FH_HIP(hipGetLastError());
+    return FH_OK;
+  }
+};
+
+}  // namespace
+}  // namespace fh
+
+extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a, const int64_t* const* offsets_a,
+                                      int64_t n_a, const uint8_t* const* chars_b, const int64_t* const* offsets_b,
+                                      int64_t n_b, const int32_t* block_a, const int32_t* block_b, int32_t n_blocks,
+                                      int n_exact, const int64_t* const* ids_a, const int64_t* const* ids_b, float p,
+                                      float lower_thr, float upper_thr, const fh_blocked_tuning* tuning, void** plan_out,
+                                      fh_stream_t stream) {
+  using namespace fh;
+  clear_error();
+  FH_REQUIRE(plan_out != nullptr, "fh_blocked_plan_create: plan is null");
+  *plan_out = nullptr;
+  FH_REQUIRE(n_fuzzy >= 1 && n_fuzzy <= kMaxBlkVars, "fh_blocked_plan_create: 1..%d fuzzy variables", kMaxBlkVars);
+  FH_REQUIRE(n_exact >= 0 && n_exact <= kMaxBlkExact, "fh_blocked_plan_create: 0..%d exact variables", kMaxBlkExact);
+  FH_REQUIRE(n_a > 0 && n_b > 0 && n_a < (1ll << 31) && n_b < (1ll << 31), "fh_blocked_plan_create: bad sizes");
+  FH_REQUIRE(n_blocks >= 1 && (int64_t)n_blocks * kNumClasses < (1ll << 31), "fh_blocked_plan_create: bad block count");
+  fh_blocked_tuning tu;
+  tu.tile_width = 256;
+  tu.tiles_per_item = 4;
+  tu.waves_per_item = kWaves;
+  tu.wgs_per_cu = 2;
+  tu.hash_slots = 0;
+  if (tuning) tu = *tuning;
+  FH_REQUIRE(tu.tile_width == 32 || tu.tile_width == 64 || tu.tile_width == 128 || tu.tile_width == 256,
+             "fh_blocked_plan_create: tile_width must be 32, 64, 128 or 256");
+  FH_REQUIRE(tu.tiles_per_item >= 1 && tu.waves_per_item >= 1 && tu.waves_per_item <= kWaves && tu.wgs_per_cu >= 1,
+             "fh_blocked_plan_create: bad tuning");
+  hipStream_t caller = (hipStream_t)stream;
+  auto pl = std::make_unique<BlockedPlan>();
+  pl->n_vars = n_fuzzy;
+  pl->n_a = n_a;
+  pl->n_b = n_b;
+  pl->n_blocks = n_blocks;
+  pl->tw = tu.tile_width;
+  FH_HIP(hipStreamCreateWithFlags(&pl->stream, hipStreamNonBlocking));
+  FH_HIP(hipEventCreateWithFlags(&pl->ev_in, hipEventDisableTiming));
+  FH_HIP(hipEventCreateWithFlags(&pl->ev_out, hipEventDisableTiming));
+  FH_HIP(hipEventCreate(&pl->ev_t0));
+  FH_HIP(hipEventCreate(&pl->ev_t1));
+  FH_HIP(hipEventRecord(pl->ev_in, caller));
+  FH_HIP(hipStreamWaitEvent(pl->stream, pl->ev_in, 0));
+  hipStream_t s = pl->stream;
+
+  const float lo = lower_thr, hi = upper_thr;
+  if (host_reaches(0.0f, lo, hi)) {
+    set_error("fh_blocked_plan_create: thresholds at or below 0 make every pair a hit; use the plain path");
+    return FH_ERR_UNSUPPORTED;
+  }
+  uint64_t n_patterns = 1;
+  for (int v = 0; v < n_fuzzy; ++v) n_patterns *= 3;
+  n_patterns <<= n_exact;
+  pl->n_patterns = n_patterns;
+  {
+    uint32_t st = 1;
+    for (int q = n_exact - 1; q >= 0; --q) {
+      pl->ex.a[q] = ids_a[q];
+      pl->ex.b[q] = ids_b[q];
+      pl->ex.stride[q] = st;
+      st *= 2;
+    }
+    pl->ex.n = n_exact;
+  }
+  pl->block_a = block_a;
+  const int nkeys_bits = bits_for((uint64_t)n_blocks * kNumClasses);
+  const int64_t nk = (int64_t)n_blocks * kNumClasses;
+  std::vector<uint8_t> mmin((kLongLen + 1) * (kLongLen + 1), 255);
+  std::vector<bool> mm_done((kLongLen + 1) * (kLongLen + 1), false);
+  std::vector<WorkItem> items[2];
+  int max_krows = 1;
+  pl->args.resize(n_fuzzy);
+
+  // Per-variable scratch for the sort (temporary) and per-variable persistent
+  // buffers (sorted rows, waves, tiles, tables).
+  for (int v = 0; v < n_fuzzy; ++v) {
+    Scratch scr(s);
+    uint32_t *key_a, *key_b, *key_tmp, *meta;
+    uint64_t *ent_a, *ent_b, *ent_tmp;
+    uint32_t *hist_a, *hist_b;
+    FH_TRY(pl->alloc(&ent_a, (size_t)n_a));
+    FH_TRY(pl->alloc(&ent_b, (size_t)n_b));
+    FH_TRY(scr.alloc(&key_a, (size_t)n_a));
+    FH_TRY(scr.alloc(&key_b, (size_t)n_b));
+    FH_TRY(scr.alloc(&key_tmp, (size_t)std::max(n_a, n_b)));
+    FH_TRY(scr.alloc(&ent_tmp, (size_t)std::max(n_a, n_b)));
+    FH_TRY(scr.alloc(&hist_a, (size_t)nk));
+    FH_TRY(scr.alloc(&hist_b, (size_t)nk));
+    FH_TRY(scr.alloc(&meta, 32));
+    FH_TRY(fill_u32(hist_a, 0, nk, s));
+    FH_TRY(fill_u32(hist_b, 0, nk, s));
+    FH_TRY(fill_u32(meta, 0, 32, s));
+    const int64_t ga = std::min<int64_t>(ceil_div(n_a, 256), 4096), gb = std::min<int64_t>(ceil_div(n_b, 256), 4096);
+    hipLaunchKernelGGL(k_blk_classify, dim3((unsigned)ga), dim3(256), 0, s, chars_a[v], offsets_a[v], n_a, block_a,
+                       key_a, ent_a, hist_a, meta, meta + 16);
+    FH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_blk_classify, dim3((unsigned)gb), dim3(256), 0, s, chars_b[v], offsets_b[v], n_b, block_b,
+                       key_b, ent_b, hist_b, meta + 8, meta + 17);
+    FH_HIP(hipGetLastError());
+    FH_TRY(radix_sort_u32(key_a, key_tmp, ent_a, ent_tmp, n_a, 0, nkeys_bits, s));
+    FH_TRY(radix_sort_u32(key_b, key_tmp, ent_b, ent_tmp, n_b, 0, nkeys_bits, s));
+    std::vector<uint32_t> ha(nk), hb(nk), hm(32);
+    FH_HIP(hipMemcpyAsync(ha.data(), hist_a, nk * 4, hipMemcpyDeviceToHost, s));
+    FH_HIP(hipMemcpyAsync(hb.data(), hist_b, nk * 4, hipMemcpyDeviceToHost, s));
+    FH_HIP(hipMemcpyAsync(hm.data(), meta, 32 * 4, hipMemcpyDeviceToHost, s));
+    FH_HIP(hipStreamSynchronize(s));
+    if (hm[16] > 0 || hm[17] > 0) {
+      set_error("fh_blocked_plan_create: strings longer than %d bytes need the plain path", kLongLen);
+      return FH_ERR_UNSUPPORTED;
+    }
+    // Alphabet of this variable.
+    std::vector<uint8_t> amap(256);
+    int K = 0;
+    for (int b = 0; b < 256; ++b)
+      if (((hm[b >> 5] >> (b & 31)) & 1u) && ((hm[8 + (b >> 5)] >> (b & 31)) & 1u)) ++K;
+    const int krows = K + 1;
+    for (int b = 0, k = 0; b < 256; ++b) {
+      const bool both = ((hm[b >> 5] >> (b & 31)) & 1u) && ((hm[8 + (b >> 5)] >> (b & 31)) & 1u);
+      amap[b] = both ? (uint8_t)k++ : (uint8_t)K;
+    }
+    max_krows = std::max(max_krows, krows);
+    // Waves and tiles per (block, length class), in sorted order.
+    std::vector<WaveMeta> waves;
+    std::vector<TileMeta> tiles;
+    std::vector<int64_t> wave_lo(n_blocks + 1), tile_lo(n_blocks + 1);
+    std::vector<int64_t> wave_short_end(n_blocks);
+    int64_t sa = 0, sb = 0;
+    for (int32_t b = 0; b < n_blocks; ++b) {
+      wave_lo[b] = (int64_t)waves.size();
+      tile_lo[b] = (int64_t)tiles.size();
+      for (int c = 0; c < kNumClasses; ++c) {
+        const int64_t ca = ha[(int64_t)b * kNumClasses + c], cb = hb[(int64_t)b * kNumClasses + c];
+        if (c == 33) wave_short_end[b] = (int64_t)waves.size();
+        if (c >= 1 && c <= kLongLen) {
+          for (int64_t s0 = sa; s0 < sa + ca; s0 += 64) {
+            WaveMeta wm;
+            wm.len = c;
+            wm.count = (int32_t)std::min<int64_t>(64, sa + ca - s0);
+            wm.s0 = s0;
+            waves.push_back(wm);
+          }
+          const int per = kind_tile_strings(kind_of_len(c), pl->tw);
+          for (int64_t s0 = sb; s0 < sb + cb; s0 += per) {
+            TileMeta tm;
+            tm.len = c;
+            tm.count = (int32_t)std::min<int64_t>(per, sb + cb - s0);
+            tm.s0 = s0;
+            tiles.push_back(tm);
+          }
+          for (int lb = 1; lb <= kLongLen && ca > 0; ++lb) {
+            if (hb[(int64_t)b * kNumClasses + lb] == 0) continue;
+            const int idx = c * (kLongLen + 1) + lb;
+            if (!mm_done[idx]) {
+              mmin[idx] = host_mmin(c, lb, p, lo, hi);
+              mm_done[idx] = true;
+            }
+            const double pairs = (double)ca * (double)hb[(int64_t)b * kNumClasses + lb];
+            if (mmin[idx] <= std::min(c, lb)) {
+              pl->pairs_kernel += pairs;
+              pl->alg_ops += pairs * kind_pair_ops(kind_of_len(lb), c);
+            }
+          }
+        }
+        sa += ca;
+        sb += cb;
+      }
+      if (wave_short_end[b] < wave_lo[b]) wave_short_end[b] = wave_lo[b];
+    }
+    wave_lo[n_blocks] = (int64_t)waves.size();
+    tile_lo[n_blocks] = (int64_t)tiles.size();
+    // Work items: per block, groups of waves (short: 1..32 bytes; long:
+    // 33..64) x chunks of tiles; cost ~ lanes x strings x length.
+    for (int32_t b = 0; b < n_blocks; ++b) {
+      const int64_t t0 = tile_lo[b], t1 = tile_lo[b + 1];
+      if (t1 == t0) continue;
+      double tile_cost = 0;
+      for (int64_t t = t0; t < t1; ++t) tile_cost += tiles[t].count;
+      for (int part = 0; part < 2; ++part) {
+        const int64_t w0 = part == 0 ? wave_lo[b] : wave_short_end[b];
+        const int64_t w1 = part == 0 ? wave_short_end[b] : wave_lo[b + 1];
+        for (int64_t w = w0; w < w1; w += tu.waves_per_item) {
+          const int nw = (int)std::min<int64_t>(tu.waves_per_item, w1 - w);
+          double lanes = 0;
+          for (int k = 0; k < nw; ++k) lanes += (double)waves[w + k].len;
+          for (int64_t t = t0; t < t1; t += tu.tiles_per_item) {
+            WorkItem it;
+            it.var = v;
+            it.nwaves = nw;
+            it.wave0 = w;
+            it.tile0 = t;
+            it.ntiles = std::min<int64_t>(tu.tiles_per_item, t1 - t);
+            it.cost = lanes * tile_cost * (double)it.ntiles / (double)(t1 - t0);
+            items[part].push_back(it);
+          }
+        }
+      }
+    }
+    // Upload this variable's plan.
+    WaveMeta* d_waves;
+    TileMeta* d_tiles;
+    uint8_t* d_small;
+    uint32_t* d_table;
+    FH_TRY(pl->alloc(&d_waves, std::max<size_t>(1, waves.size())));
+    FH_TRY(pl->alloc(&d_tiles, std::max<size_t>(1, tiles.size())));
+    FH_TRY(pl->alloc(&d_small, 256 + mmin.size()));
+    FH_TRY(pl->alloc(&d_table, std::max<size_t>(1, tiles.size()) * krows * pl->tw));
+    if (!waves.empty())
+      FH_HIP(hipMemcpyAsync(d_waves, waves.data(), waves.size() * sizeof(WaveMeta), hipMemcpyHostToDevice, s));
+    if (!tiles.empty())
+      FH_HIP(hipMemcpyAsync(d_tiles, tiles.data(), tiles.size() * sizeof(TileMeta), hipMemcpyHostToDevice, s));
+    FH_HIP(hipMemcpyAsync(d_small, amap.data(), 256, hipMemcpyHostToDevice, s));
+    FH_HIP(hipStreamSynchronize(s));
+    JwArgs& a = pl->args[v];
+    a.chars_a = chars_a[v];
+    a.off_a = offsets_a[v];
+    a.sorted_a = ent_a;
+    a.chars_b = chars_b[v];
+    a.off_b = offsets_b[v];
+    a.sorted_b = ent_b;
+    a.waves = d_waves;
+    a.tiles = d_tiles;
+    a.table = d_table;
+    a.amap = d_small;
+    a.mmin = d_small + 256;  // filled once every variable's lengths are known
+    a.krows = krows;
+    a.tw = pl->tw;
+    a.mode = FH_MODE_LINK;
+    a.n_b = n_b;
+    a.p = p;
+    a.lo = lo;
+    a.hi = hi;
+    a.hits = nullptr;
+    a.cap = 0;
+    a.counter = nullptr;
+    a.code_stride = 1;
+    for (int q = v + 1; q < n_fuzzy; ++q) a.code_stride *= 3;
+    a.code_stride <<= n_exact;
+    pl->chars_b.push_back(chars_b[v]);
+    pl->off_b.push_back(offsets_b[v]);
+    pl->sorted_b.push_back(ent_b);
+    pl->tiles_dev.push_back(d_tiles);
+    pl->ntiles.push_back((int64_t)tiles.size());
+    pl->amap_dev.push_back(d_small);
+    pl->krows.push_back(krows);
+    pl->table.push_back(d_table);
+  }
+  // Pruning table (shared by the variables), hash table, counters.
+  for (int v = 0; v < n_fuzzy; ++v)
+    FH_HIP(hipMemcpyAsync(pl->amap_dev[v] + 256, mmin.data(), mmin.size(), hipMemcpyHostToDevice, s));
+  uint64_t want = tu.hash_slots > 0 ? (uint64_t)tu.hash_slots : (uint64_t)std::max<int64_t>(1 << 22, 2 * (n_a + n_b));
+  uint64_t slots = 1;
+  while (slots < want) slots <<= 1;
+  pl->hslots = slots;
+  FH_TRY(pl->alloc(&pl->hkeys, slots));
+  FH_TRY(pl->alloc(&pl->hcodes, slots));
+  FH_TRY(pl->alloc(&pl->heads, 2));
+  FH_TRY(pl->alloc(&pl->overflow, 1));
+  FH_TRY(pl->alloc(&pl->hist, (size_t)(n_patterns * (uint64_t)n_blocks)));
+  FH_TRY(pl->alloc(&pl->n_union, 1));
+  for (auto& a : pl->args) {
+    a.hkeys = pl->hkeys;
+    a.hcodes = pl->hcodes;
+    a.hmask = slots - 1;
+    a.overflow = pl->overflow;
+    a.krows_max = max_krows;
+  }
+  FH_TRY(pl->alloc(&pl->args_dev, pl->args.size()));
+  FH_HIP(hipMemcpyAsync(pl->args_dev, pl->args.data(), pl->args.size() * sizeof(JwArgs), hipMemcpyHostToDevice, s));
+  for (int part = 0; part < 2; ++part) {
+    std::stable_sort(items[part].begin(), items[part].end(),
+                     [](const WorkItem& x, const WorkItem& y) { return x.cost > y.cost; });
+    pl->n_items[part] = (int64_t)items[part].size();
+    FH_TRY(pl->alloc(&pl->items_dev[part], std::max<size_t>(1, items[part].size())));
+    if (!items[part].empty())
+      FH_HIP(hipMemcpyAsync(pl->items_dev[part], items[part].data(), items[part].size() * sizeof(WorkItem),
+                            hipMemcpyHostToDevice, s));
+  }
+  const size_t qbytes = (size_t)kWaves * kQCap * sizeof(uint64_t);
+  pl->lds_bytes = ((size_t)max_krows * (pl->tw + 1) + 1) * 4 + qbytes;
+  FH_REQUIRE(pl->lds_bytes <= 64 * 1024, "fh_blocked_plan_create: alphabet too large for tile width %d", pl->tw);
+  int dev = 0, cus = 0;
+  FH_HIP(hipGetDevice(&dev));
+  FH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  pl->grid = std::max(1, cus * tu.wgs_per_cu);
+  FH_HIP(hipStreamSynchronize(s));
+  *plan_out = pl.release();
+  return FH_OK;
+}
+
+extern "C" int fh_blocked_run(void* plan, int use_graph, fh_stream_t stream) {
+  using namespace fh;
+  clear_error();
+  FH_REQUIRE(plan != nullptr, "fh_blocked_run: plan is null");
+  BlockedPlan* pl = static_cast<BlockedPlan*>(plan);
+  hipStream_t caller = (hipStream_t)stream;
+  FH_HIP(hipEventRecord(pl->ev_in, caller));
+  FH_HIP(hipStreamWaitEvent(pl->stream, pl->ev_in, 0));
+  FH_HIP(hipEventRecord(pl->ev_t0, pl->stream));
+  if (use_graph) {
+    if (!pl->exec) {
+      FH_HIP(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal));
+      const int st = pl->enqueue();
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(pl->stream, &g);
+      if (st != FH_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+      }
+      FH_HIP(e);
+      pl->graph = g;
+      FH_HIP(hipGraphInstantiate(&pl->exec, pl->graph, nullptr, nullptr, 0));
+    }
+    FH_HIP(hipGraphLaunch(pl->exec, pl->stream));
+  } else {
+    FH_TRY(pl->enqueue());
+  }
+  FH_HIP(hipEventRecord(pl->ev_t1, pl->stream));
+  FH_HIP(hipEventRecord(pl->ev_out, pl->stream));
+  FH_HIP(hipStreamWaitEvent(caller, pl->ev_out, 0));
+  return FH_OK;
+}
+
+extern "C" int fh_blocked_counts(void* plan, int64_t* hist, int64_t* n_union, int* overflow, double* ms) {
+  using namespace fh;
+  clear_error();
+  FH_REQUIRE(plan != nullptr, "fh_blocked_counts: plan is null");
+  BlockedPlan* pl = static_cast<BlockedPlan*>(plan);
+  FH_HIP(hipStreamSynchronize(pl->stream));
+  if (hist)
+    FH_HIP(hipMemcpy(hist, pl->hist, pl->n_patterns * (uint64_t)pl->n_blocks * sizeof(int64_t), hipMemcpyDeviceToHost));
+  uint64_t nu = 0;
+  unsigned int ov = 0;
+  FH_HIP(hipMemcpy(&nu, pl->n_union, sizeof(nu), hipMemcpyDeviceToHost));
+  FH_HIP(hipMemcpy(&ov, pl->overflow, sizeof(ov), hipMemcpyDeviceToHost));
+  if (n_union) *n_union = (int64_t)nu;
+  if (overflow) *overflow = (int)ov;
+  if (ms) {
+    float t = 0;
+    FH_HIP(hipEventElapsedTime(&t, pl->ev_t0, pl->ev_t1));
+    *ms = t;
+  }
+  return FH_OK;
+}
+
+extern "C" int fh_blocked_info(void* plan, double* info) {
+  using namespace fh;
+  clear_error();
+  FH_REQUIRE(plan != nullptr && info != nullptr, "fh_blocked_info: null argument");
+  BlockedPlan* pl = static_cast<BlockedPlan*>(plan);
+  info[0] = (double)pl->n_items[0];
+  info[1] = (double)pl->n_items[1];
+  info[2] = (double)pl->n_patterns;
+  info[3] = (double)pl->hslots;
+  info[4] = pl->pairs_kernel;
+  info[5] = pl->alg_ops;
+  info[6] = (double)pl->grid;
+  info[7] = (double)pl->lds_bytes;
+  return FH_OK;
+}
+
+extern "C" int fh_blocked_plan_destroy(void* plan) {
+  fh::clear_error();
+  delete static_cast<fh::BlockedPlan*>(plan);
+  return FH_OK;
+}

@@ -1,0 +1,761 @@
+// Device code of the all-pairs Jaro-Winkler comparison, shared by the
+// translation units that instantiate it (fh_jw_r0/r1/r2.hip: the plain
+// comparison per string-1 length range; fh_jw_blocked.hip: the persistent
+// blocked form; fh_jw.hip: plan and entry point).
+//
+// All-pairs Jaro-Winkler levels on gfx950.
+//
+// Replaces the reference's comparison kernel (faster/comparison.py:133-186,
+// deduplication copy faster/deduplication.py:132-196) and its thresholding
+// and compaction (faster/comparison.py:321-338).
+//
+// Design (one GPU, 64-wide wavefronts, VALU-bound):
+//   * Strings of 1..64 bytes are sorted by length. A wavefront holds 64
+//     strings of one length ("string 1", one per lane); every length-derived
+//     quantity of a pair (match window, pruning bound) is then uniform across
+//     the wave and lives in scalar registers.
+//   * Strings of side B ("string 2") are grouped into tiles of one length and
+//     pre-encoded as complemented per-character position masks:
+//     nmask[c][b] has bit j clear iff byte j of string b has alphabet code c.
+//     A workgroup stages one tile in LDS (row stride = tile width + 1 dwords,
+//     so the 64 lanes' rows fall in distinct banks) and streams it against its
+//     8 waves of string 1.
+//   * The reference's greedy first-unused match (faster/comparison.py:48-64)
+//     becomes, per character of string 1, three VALU ops on a mask of still
+//     unmatched positions of string 2 ("navail", complemented):
+//         nc = nmask | nwin_i | navail        (v_bitop3: candidates = zeros)
+//         t  = nc + 1                          (lowest candidate -> 1)
+//         navail = (nc & navail) | (~nc & t)   (v_bitop3: mark it used)
+//     Short string-2 masks are packed several to a dword, each in a lane of
+//     LB bits whose bit 0 is a guard that is always set in nc: lanes of 8
+//     bits (strings of <= 7 bytes, 4 per dword), 10 bits (<= 9 bytes, 3 per
+//     dword) or 16 bits (<= 15 bytes, 2 per dword). One 32-bit add of 1 per
+//     lane then advances every lane at once: a lane whose bits are all set
+//     carries into the next lane's guard, which absorbs it (the guard is set
+//     in nc, so the select never copies it), and no carry reaches a
+//     position bit of another lane.
+//   * The match count m = popcount of used positions is an exact count for the
+//     greedy match. A host-built table gives, for every (len1, len2), the
+//     smallest m for which any transposition count and prefix can reach level
+//     1 in the reference's fp32 arithmetic; pairs below it are level 0 and
+//     cost nothing more. Survivors are queued per wave in LDS and evaluated
+//     with all 64 lanes by an exact restatement of the reference function.
+//   * Pairs with an empty string (only when a score of 0 reaches level 1) or a
+//     string longer than 64 bytes go through a generic kernel.
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "fh_common.h"
+
+namespace fh {
+
+constexpr int kWaves = 8;             // waves per workgroup
+constexpr int kThreads = kWaves * 64;
+constexpr int kU = 8;                 // dword columns per inner step
+constexpr int kQCap = 128;            // survivor queue entries per wave
+constexpr int kLongLen = 64;          // longest string on the bit-parallel path
+constexpr int kMaxGenericLen = 1024;  // longest string supported at all
+constexpr int kClassLong = kLongLen + 1;
+constexpr int kNumClasses = kLongLen + 2;  // 0 = empty, 1..64, 65 = long
+
+// String-2 mask layouts ("kinds"), chosen by string-2 length.
+enum Kind { kP4 = 0, kP3 = 1, kP2 = 2, kW32 = 3, kW64 = 4 };
+constexpr int kNumKinds = 5;
+
+__host__ __device__ constexpr bool packed_kind(int k) { return k <= kP2; }
+// Strings per dword of a packed kind and the width of its lanes.
+__host__ __device__ constexpr int kind_lanes(int k) { return k == kP4 ? 4 : (k == kP3 ? 3 : 2); }
+__host__ __device__ constexpr int kind_lane_bits(int k) { return k == kP4 ? 8 : (k == kP3 ? 10 : 16); }
+// Bit 0 of every lane (the guard, and the place the per-lane +1 enters).
+__host__ __device__ constexpr uint32_t kind_ones(int k) {
+  return k == kP4 ? 0x01010101u : (k == kP3 ? 0x00100401u : 0x00010001u);
+}
+// Bits above the last lane (kP3 only): permanently blocked.
+__host__ __device__ constexpr uint32_t kind_top(int k) { return k == kP3 ? 0xC0000000u : 0u; }
+__host__ __device__ constexpr int kind_of_len(int len) {
+  return len <= 7 ? kP4 : (len <= 9 ? kP3 : (len <= 15 ? kP2 : (len <= 32 ? kW32 : kW64)));
+}
+// Strings of one tile of `tw` dword columns.
+__host__ __device__ constexpr int kind_tile_strings(int k, int tw) {
+  return packed_kind(k) ? kind_lanes(k) * tw : (k == kW32 ? tw : tw / 2);
+}
+// Algorithmic VALU lane-ops per pair of string-1 length la: three ops per
+// character and dword (or3, add, select) plus the survivor test (extract and
+// count each packed string, one compare per dword), divided by the strings
+// sharing the dword; 64-bit masks take two dwords per string.
+inline double kind_pair_ops(int k, int la) {
+  if (packed_kind(k)) return (3.0 * la + 2.0 * kind_lanes(k) + 1.0) / kind_lanes(k);
+  return k == kW32 ? 3.0 * la + 2.0 : 6.0 * la + 4.0;
+}
+
+struct WaveMeta {
+  int32_t len;    // string-1 length of every lane
+  int32_t count;  // valid lanes (1..64)
+  int64_t s0;     // first sorted index
+};
+
+struct TileMeta {
+  int32_t len;    // string-2 length of every string in the tile
+  int32_t count;  // strings in the tile
+  int64_t s0;     // first sorted index
+};
+
+struct JwArgs {
+  const uint8_t* chars_a;
+  const int64_t* off_a;
+  const uint64_t* sorted_a;
+  const uint8_t* chars_b;
+  const int64_t* off_b;
+  const uint64_t* sorted_b;
+  const WaveMeta* waves;
+  const TileMeta* tiles;
+  const uint32_t* table;  // [tile][krows][tw]
+  const uint8_t* amap;    // byte -> alphabet code (krows - 1 = no match)
+  const uint8_t* mmin;    // [65][65]
+  int krows;
+  int tw;
+  int mode;
+  int64_t n_b;  // id stride (n_b for linkage, n for deduplication)
+  float p, lo, hi;
+  uint64_t* hits;
+  int64_t cap;
+  unsigned long long* counter;
+  int krows_max;  // LDS rows reserved (>= krows)
+  // Hash emission (blocked linkage): when hkeys is set, a hit adds
+  // level * code_stride to the code of its pair id in an open-addressing
+  // table instead of being appended to `hits`.
+  unsigned long long* hkeys;
+  uint32_t* hcodes;
+  uint64_t hmask;
+  uint32_t code_stride;
+  unsigned int* overflow;
+};
+
+constexpr unsigned long long kEmptyKey = ~0ull;
+constexpr int kMaxProbes = 4096;
+
+// Host side: the pruning bound.
+inline float host_formula(int l1, int l2, int m, int tt, int pre, float p, bool fma) {
+  if (m == 0) return 0.0f;
+  float match = (float)m;
+  float t = (float)tt;
+  t /= 2;
+  float dist = (float)((double)((match / (float)l1) + (match / (float)l2) + ((match - t) / match)) / 3.0);
+  float prefix = (float)pre;
+  if (fma) return std::fmaf(p * prefix, 1 - dist, dist);
+  volatile float q = (p * prefix) * (1 - dist);
+  return dist + q;
+}
+
+inline bool host_reaches(float s, float lo, float hi) { return (s >= hi) || (s >= lo && s < hi); }
+
+// Smallest match count for which some transposition count and prefix reach
+// level >= 1 (255: unreachable). Exhaustive over (tt, prefix) per m, both
+// with and without the fused final update, so the bound is conservative.
+inline uint8_t host_mmin(int l1, int l2, float p, float lo, float hi) {
+  if (host_reaches(0.0f, lo, hi)) return 0;
+  const int mmax = std::min(l1, l2);
+  const int pmax = std::min(4, mmax);
+  for (int m = 1; m <= mmax; ++m)
+    for (int tt = 0; tt <= m; ++tt)
+      for (int pre = 0; pre <= pmax; ++pre)
+        if (host_reaches(host_formula(l1, l2, m, tt, pre, p, true), lo, hi) ||
+            host_reaches(host_formula(l1, l2, m, tt, pre, p, false), lo, hi))
+          return (uint8_t)m;
+  return 255;
+}
+
+
+// Launches k_jw_tiles over waves [w0, w0 + nw) of one string-1 length range
+// (r0: 1..16 bytes, r1: 17..32, r2: 33..64) against tiles [t0, t0 + nt) of
+// one kind. Each range lives in its own translation unit.
+int launch_jw_tiles_r0(int kind, const JwArgs* args, size_t lds_bytes, int64_t w0, int64_t nw, int64_t t0, int64_t nt,
+                       hipStream_t s);
+int launch_jw_tiles_r1(int kind, const JwArgs* args, size_t lds_bytes, int64_t w0, int64_t nw, int64_t t0, int64_t nt,
+                       hipStream_t s);
+int launch_jw_tiles_r2(int kind, const JwArgs* args, size_t lds_bytes, int64_t w0, int64_t nw, int64_t t0, int64_t nt,
+                       hipStream_t s);
+
+// Device code: internal to every translation unit that includes this header.
+namespace {
+
+// ---------------------------------------------------------------------------
+// The reference's score, restated for the device (faster/comparison.py:11-131)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float jw_formula(int l1, int l2, int m, int tt, int pre, float p) {
+  if (m == 0) return 0.0f;
+  float match = (float)m;
+  float t = (float)tt;
+  t /= 2;
+  float dist = (float)((double)((match / (float)l1) + (match / (float)l2) + ((match - t) / match)) / 3.0);
+  float prefix = (float)pre;
+  return fmaf(p * prefix, 1 - dist, dist);
+}
+
+__device__ __forceinline__ int level_of(float s, float lo, float hi) {
+  if (s >= hi) return 2;
+  if (s >= lo && s < hi) return 1;
+  return 0;
+}
+
+// Greedy match with per-position "used" bits held in WORDS 64-bit words
+// (strings up to WORDS * 64 bytes), transpositions, prefix and formula.
+template <int WORDS>
+__device__ float jw_exact(const uint8_t* __restrict__ s1, int l1, const uint8_t* __restrict__ s2, int l2,
+                          float p) {
+  if (l1 == 0 || l2 == 0) return 0.0f;
+  const int w = max(l1, l2) / 2 - 1;
+  uint64_t used2[WORDS];
+  uint64_t flag1[WORDS];
+#pragma unroll
+  for (int k = 0; k < WORDS; ++k) {
+    used2[k] = 0;
+    flag1[k] = 0;
+  }
+  int m = 0;
+  for (int i = 0; i < l1; ++i) {
+    const uint8_t c = s1[i];
+    const int jlo = max(0, i - w);
+    const int jhi = min(l2, i + w + 1);
+    for (int j = jlo; j < jhi; ++j) {
+      if (s2[j] == c && !((used2[j >> 6] >> (j & 63)) & 1ull)) {
+        used2[j >> 6] |= 1ull << (j & 63);
+        flag1[i >> 6] |= 1ull << (i & 63);
+        ++m;
+        break;
+      }
+    }
+  }
+  if (m == 0) return 0.0f;
+  int tt = 0;
+  int wi = 0, wj = 0;
+  uint64_t f = flag1[0], u = used2[0];
+  for (int k = 0; k < m; ++k) {
+    while (f == 0) f = flag1[++wi];
+    while (u == 0) u = used2[++wj];
+    const int i = wi * 64 + __builtin_ctzll(f);
+    const int j = wj * 64 + __builtin_ctzll(u);
+    if (s1[i] != s2[j]) ++tt;
+    f &= f - 1;
+    u &= u - 1;
+  }
+  int pre = 0;
+  const int lim = min(min(l1, l2), 4);
+  for (int i = 0; i < lim; ++i) {
+    if (s1[i] == s2[i])
+      ++pre;
+    else
+      break;
+  }
+  return jw_formula(l1, l2, m, tt, pre, p);
+}
+
+// Byte-wise lexicographic order (UTF-8 byte order equals code-point order);
+// returns true when (s1, l1) > (s2, l2).
+__device__ __forceinline__ bool lex_greater(const uint8_t* s1, int l1, const uint8_t* s2, int l2) {
+  const int n = min(l1, l2);
+  for (int i = 0; i < n; ++i) {
+    if (s1[i] != s2[i]) return s1[i] > s2[i];
+  }
+  return l1 > l2;
+}
+
+
+__device__ __forceinline__ uint64_t hash_id(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+// Adds `add` to the code of pair `id` (linear probing, bounded; a full
+// neighbourhood raises the overflow flag and drops the update).
+__device__ __forceinline__ void hash_add(const JwArgs& a, uint64_t id, uint32_t add) {
+  uint64_t slot = hash_id(id) & a.hmask;
+  for (int probe = 0; probe < kMaxProbes; ++probe) {
+    const unsigned long long prev = atomicCAS(a.hkeys + slot, kEmptyKey, (unsigned long long)id);
+    if (prev == kEmptyKey || prev == id) {
+      atomicAdd(a.hcodes + slot, add);
+      return;
+    }
+    slot = (slot + 1) & a.hmask;
+  }
+  atomicOr(a.overflow, 1u);
+}
+
+// Evaluates one queued pair exactly and emits its key when its level is > 0.
+__device__ __noinline__ void eval_pair(const JwArgs& a, uint64_t ea, uint64_t eb) {
+  const int64_t oa = a.off_a[ea];
+  const int la = (int)(a.off_a[ea + 1] - oa);
+  const int64_t ob = a.off_b[eb];
+  const int lb = (int)(a.off_b[eb + 1] - ob);
+  const uint8_t* sa = a.chars_a + oa;
+  const uint8_t* sb = a.chars_b + ob;
+  uint64_t x = ea, y = eb;
+  float s;
+  if (a.mode == FH_MODE_DEDUP && !lex_greater(sa, la, sb, lb)) {
+    s = jw_exact<1>(sb, lb, sa, la, a.p);
+    x = eb;
+    y = ea;
+  } else {
+    s = jw_exact<1>(sa, la, sb, lb, a.p);
+  }
+  const int lvl = level_of(s, a.lo, a.hi);
+  if (lvl > 0) {
+    const uint64_t id = (uint64_t)x * (uint64_t)a.n_b + y;
+    if (a.hkeys) {
+      hash_add(a, id, (uint32_t)lvl * a.code_stride);
+    } else {
+      const unsigned long long pos = atomicAdd(a.counter, 1ull);
+      if ((int64_t)pos < a.cap) a.hits[pos] = id * 4ull + (uint64_t)lvl;
+    }
+  }
+}
+
+// Processes the wave's queue of (string-1 lane, string-2 sorted index) pairs.
+__device__ __noinline__ void flush_queue(const JwArgs& a, const uint64_t* q, int qn, uint64_t s0a) {
+  for (int k = (int)__lane_id(); k < qn; k += 64) {
+    const uint64_t v = q[k];
+    const uint64_t ea = a.sorted_a[s0a + (v & 63ull)];
+    const uint64_t eb = a.sorted_b[v >> 6];
+    eval_pair(a, ea, eb);
+  }
+}
+
+// Pushes one survivor per lane where `pred` holds; flushes first when full.
+__device__ __forceinline__ void queue_push(const JwArgs& a, uint64_t* q, int& qn, bool pred, uint64_t sb,
+                                           uint64_t s0a) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  if (qn + 64 > kQCap) {
+    flush_queue(a, q, qn, s0a);
+    qn = 0;
+  }
+  if (pred) {
+    const int pos = qn + __popcll(mask & fhd::lanemask_lt());
+    q[pos] = (sb << 6) | (uint64_t)__lane_id();
+  }
+  qn += __popcll(mask);
+}
+
+// gfx950 v_bitop3_b32 forms used by the match recurrence. The truth-table
+// index of bit k is (a_k << 2) | (b_k << 1) | c_k.
+__device__ __forceinline__ uint32_t or3(uint32_t a, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0xFE);  // a | b | c
+}
+__device__ __forceinline__ uint32_t select_bits(uint32_t sel, uint32_t b, uint32_t c) {
+  return __builtin_amdgcn_bitop3_b32(sel, b, c, 0xCA);  // sel ? b : c, bitwise
+}
+// Per-wave state for one (wave, tile) pass.
+struct PassCtx {
+  const uint32_t* lds;  // staged tile, row stride tw + 1 dwords
+  uint64_t* q;          // this wave's survivor queue
+  int qn;
+  int lenB, cntB, mm, w;
+  int64_t s0a, s0b;
+  bool lane_ok, diag;
+};
+
+__device__ __forceinline__ const uint32_t* row_ptr(const uint32_t* lds, uint32_t off_bytes, int col) {
+  return reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(lds) + off_bytes) + col;
+}
+
+// Window of string-2 positions [i - w, i + w] clipped to [0, lenB).
+__device__ __forceinline__ uint64_t window_bits(int i, int w, int lenB) {
+  const int lo_i = max(0, i - w);
+  const int hi_i = min(lenB - 1, i + w);
+  if (hi_i < lo_i) return 0ull;
+  const int width = hi_i - lo_i + 1;
+  return (width >= 64 ? ~0ull : ((1ull << width) - 1ull)) << lo_i;
+}
+
+// Complemented window of string-1 character i in every lane of a packed
+// kind: lane bits 1..lenB are the positions, the guard (bit 0) and the bits
+// past the last lane are set.
+template <int KIND>
+__device__ __forceinline__ uint32_t packed_nwin(int i, int w, int lenB) {
+  constexpr int LB = kind_lane_bits(KIND);
+  constexpr uint32_t lane_mask = (1u << LB) - 1u;
+  const uint32_t lane = (~((uint32_t)window_bits(i, w, lenB) << 1)) & lane_mask;
+  uint32_t v = kind_top(KIND);
+#pragma unroll
+  for (int k = 0; k < kind_lanes(KIND); ++k) v |= lane << (k * LB);
+  return v;
+}
+
+// Match count of lane k of a packed dword. The guards and the bits above the
+// last lane are never set in `na` (they are set in nc, so the select keeps
+// them), so the last lane needs no mask.
+template <int KIND>
+__device__ __forceinline__ int lane_count(uint32_t na, int k) {
+  constexpr int LB = kind_lane_bits(KIND);
+  if (k == kind_lanes(KIND) - 1) return __popc(na >> (k * LB));
+  return __popc((na >> (k * LB)) & ((1u << LB) - 1u));
+}
+
+// Queues the candidates whose bit is set in `smask` (bit k = string-2 tile
+// index first + k), after the validity and triangle checks. Rare path.
+__device__ __forceinline__ void push_candidates(const JwArgs& a, PassCtx& c, uint32_t smask, int first, int n) {
+  const int lane = __lane_id();
+#pragma unroll 1
+  for (int k = 0; k < n; ++k) {
+    const int sidx = first + k;
+    bool pred = c.lane_ok && ((smask >> k) & 1u) && sidx < c.cntB;
+    if (c.diag) pred = pred && (c.s0a + lane > c.s0b + sidx);
+    queue_push(a, c.q, c.qn, pred, (uint64_t)(c.s0b + sidx), (uint64_t)c.s0a);
+  }
+}
+
+// One string-1 length LA (<= 32) against a staged tile of 16- or 32-bit
+// masks, fully unrolled over the LA characters so the row offsets stay in
+// fixed VGPRs and the windows in SGPRs.
+template <int LA, int KIND, int LMAX>
+__device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uint32_t (&off)[LMAX]) {
+  const int lenB = c.lenB;
+  if constexpr (packed_kind(KIND)) {
+    constexpr int NL = kind_lanes(KIND);
+    constexpr uint32_t ONES = kind_ones(KIND);
+    uint32_t nwin[LA];
+#pragma unroll
+    for (int i = 0; i < LA; ++i) nwin[i] = __builtin_amdgcn_readfirstlane(packed_nwin<KIND>(i, c.w, lenB));
+    // navail starts at 0: positions past lenB never become candidates because
+    // the tile masks and the windows block them, so the lane counts are m.
+    const uint32_t init = 0u;
+    const int mm = c.mm;
+    const int ncols = (c.cntB + NL - 1) / NL;
+    for (int col = 0; col < ncols; col += kU) {
+      uint32_t na[kU], cur[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        na[u] = init;
+        cur[u] = row_ptr(c.lds, off[0], col)[u];
+      }
+#pragma unroll
+      for (int i = 0; i < LA; ++i) {
+        // Rows of character i + 1 are read while character i is processed
+        // (the scheduling fences keep the reads ahead of this step's ops).
+        uint32_t nxt[kU];
+        if (i + 1 < LA) {
+#pragma unroll
+          for (int u = 0; u < kU; ++u) nxt[u] = row_ptr(c.lds, off[i + 1], col)[u];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        uint32_t nc[kU], t[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) nc[u] = or3(cur[u], nwin[i], na[u]);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) t[u] = nc[u] + ONES;
+#pragma unroll
+        for (int u = 0; u < kU; ++u) na[u] = select_bits(nc[u], na[u], t[u]);
+        __builtin_amdgcn_sched_barrier(0);
+        if (i + 1 < LA) {
+#pragma unroll
+          for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
+        }
+      }
+      // Largest match count over the NL * kU pairs.
+      uint32_t mx = 0;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+#pragma unroll
+        for (int k = 0; k < NL; ++k) mx = max(mx, (uint32_t)lane_count<KIND>(na[u], k));
+      }
+      if (__ballot((int)mx >= mm && c.lane_ok) == 0) continue;
+      uint32_t smask = 0;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+#pragma unroll
+        for (int k = 0; k < NL; ++k) smask |= (uint32_t)(lane_count<KIND>(na[u], k) >= mm) << (NL * u + k);
+      }
+      push_candidates(a, c, smask, NL * col, NL * kU);
+    }
+  } else {
+    uint32_t nwin[LA];
+#pragma unroll
+    for (int i = 0; i < LA; ++i) nwin[i] = __builtin_amdgcn_readfirstlane(~(uint32_t)window_bits(i, c.w, lenB));
+    const uint32_t init = 0u;
+    const int mm = c.mm;
+    for (int col = 0; col < c.cntB; col += kU) {
+      uint32_t na[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) na[u] = init;
+#pragma unroll
+      for (int i = 0; i < LA; ++i) {
+        const uint32_t* row = row_ptr(c.lds, off[i], col);
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const uint32_t nc = or3(row[u], nwin[i], na[u]);
+          na[u] = select_bits(nc, na[u], nc + 1u);
+        }
+      }
+      uint32_t smask = 0;
+#pragma unroll
+      for (int u = 0; u < kU; ++u) smask |= (uint32_t)((int)__popc(na[u]) >= mm) << u;
+      if (__ballot(smask != 0 && c.lane_ok) == 0) continue;
+      push_candidates(a, c, smask, col, kU);
+    }
+  }
+}
+
+// Any string-1 length (rolled loop over its characters) against any mask
+// width; used for the rare long strings (33..64 bytes on either side).
+template <int KIND, int LMAX>
+__device__ __forceinline__ void tile_pass_rolled(const JwArgs& a, PassCtx& c, const uint32_t (&off)[LMAX], int lenA) {
+  const int lenB = c.lenB;
+  constexpr int per = packed_kind(KIND) ? kind_lanes(KIND) : 1;  // strings per dword
+  constexpr int dws = KIND == kW64 ? 2 : 1;                      // dwords per string
+  const int ncols = packed_kind(KIND) ? (c.cntB + per - 1) / per : c.cntB * dws;
+#pragma unroll 1
+  for (int col = 0; col < ncols; col += dws) {
+    uint64_t na = 0;  // lane counts = m (see tile_pass)
+#pragma unroll 1
+    for (int i = 0; i < lenA; ++i) {
+      uint32_t oi = 0;
+#pragma unroll
+      for (int k = 0; k < LMAX; ++k)
+        if (k == i) oi = off[k];
+      const uint32_t* row = row_ptr(c.lds, oi, col);
+      if constexpr (packed_kind(KIND)) {
+        const uint32_t nc = row[0] | packed_nwin<KIND>(i, c.w, lenB) | (uint32_t)na;
+        na = select_bits(nc, (uint32_t)na, nc + kind_ones(KIND));
+      } else if constexpr (KIND == kW32) {
+        const uint32_t nc = row[0] | ~(uint32_t)window_bits(i, c.w, lenB) | (uint32_t)na;
+        na = (nc & (uint32_t)na) | (~nc & (nc + 1u));
+      } else {
+        const uint64_t nm = ((uint64_t)row[1] << 32) | row[0];
+        const uint64_t nc = nm | ~window_bits(i, c.w, lenB) | na;
+        na = (nc & na) | (~nc & (nc + 1ull));
+      }
+    }
+    uint32_t smask = 0;
+    if constexpr (packed_kind(KIND)) {
+#pragma unroll
+      for (int k = 0; k < per; ++k) smask |= (uint32_t)(lane_count<KIND>((uint32_t)na, k) >= c.mm) << k;
+    } else if constexpr (KIND == kW32) {
+      smask = (uint32_t)((int)__popc((uint32_t)na) >= c.mm);
+    } else {
+      smask = (uint32_t)((int)__popcll(na) >= c.mm);
+    }
+    if (__ballot(smask != 0 && c.lane_ok) == 0) continue;
+    push_candidates(a, c, smask, col * per / dws, per);
+  }
+}
+
+// Uniform dispatch from the wave's string-1 length to its unrolled pass.
+template <int LO, int HI, int KIND, int LMAX>
+struct PassDispatch {
+  __device__ static __forceinline__ void run(int lenA, const JwArgs& a, PassCtx& c, const uint32_t (&off)[LMAX]) {
+    if (lenA == LO)
+      tile_pass<LO, KIND, LMAX>(a, c, off);
+    else
+      PassDispatch<LO + 1, HI, KIND, LMAX>::run(lenA, a, c, off);
+  }
+};
+template <int HI, int KIND, int LMAX>
+struct PassDispatch<HI, HI, KIND, LMAX> {
+  __device__ static __forceinline__ void run(int lenA, const JwArgs& a, PassCtx& c, const uint32_t (&off)[LMAX]) {
+    if (lenA == HI) tile_pass<HI, KIND, LMAX>(a, c, off);
+  }
+};
+
+// Compares this workgroup's waves [wave_first, wave_first + kWaves) ∩
+// [.., wave_end) against the string-2 tiles [t_begin, t_end).
+// KIND >= 0: every tile has that mask kind. KIND == kMixed: tiles of any
+// kind (per-tile uniform dispatch), used by the persistent blocked kernel.
+constexpr int kMixed = -1;
+
+template <int LMIN, int LMAX, int KIND>
+__device__ __forceinline__ void run_item(const JwArgs& a, uint32_t* lds, int64_t wave_first, int64_t wave_end,
+                                         int64_t t_begin, int64_t t_end) {
+  const int tw = a.tw;
+  const int stride = tw + 1;
+  uint64_t* queue = reinterpret_cast<uint64_t*>(lds + a.krows_max * stride + (a.krows_max * stride & 1));
+  const int wave = threadIdx.x >> 6;
+  const int lane = __lane_id();
+
+  const int64_t wid = wave_first + wave;
+  const bool wave_ok = wid < wave_end;
+  WaveMeta wm;
+  wm.len = 0;
+  wm.count = 0;
+  wm.s0 = 0;
+  if (wave_ok) wm = a.waves[wid];
+  const int lenA = __builtin_amdgcn_readfirstlane(wm.len);
+  const int cntA = __builtin_amdgcn_readfirstlane(wm.count);
+
+  PassCtx c;
+  c.lds = lds;
+  c.q = queue + wave * kQCap;
+  c.qn = 0;
+  c.lane_ok = lane < cntA;
+  c.s0a = wm.s0;
+
+  // Byte offsets of each character's LDS row.
+  uint32_t off[LMAX];
+  {
+    int64_t base = 0;
+    if (c.lane_ok) base = a.off_a[a.sorted_a[wm.s0 + lane]];
+    const uint32_t none = (uint32_t)(a.krows - 1);
+#pragma unroll
+    for (int i = 0; i < LMAX; ++i) {
+      uint32_t code = none;
+      if (c.lane_ok && i < lenA) code = a.amap[a.chars_a[base + i]];
+      off[i] = code * (uint32_t)stride * 4u;
+    }
+  }
+
+  for (int64_t t = t_begin; t < t_end; ++t) {
+    __syncthreads();
+    {
+      const uint32_t* src = a.table + (size_t)t * a.krows * tw;
+      const int total = a.krows * tw;
+      const int shift = __builtin_ctz(tw);
+      for (int idx = threadIdx.x; idx < total; idx += kThreads) {
+        const int r = idx >> shift;
+        const int col = idx & (tw - 1);
+        lds[r * stride + col] = src[idx];
+      }
+    }
+    __syncthreads();
+    if (!wave_ok) continue;
+    const TileMeta tm = a.tiles[t];
+    c.lenB = __builtin_amdgcn_readfirstlane(tm.len);
+    c.cntB = __builtin_amdgcn_readfirstlane(tm.count);
+    c.s0b = tm.s0;
+    c.diag = false;
+    if (a.mode == FH_MODE_DEDUP) {
+      if (lenA < c.lenB) continue;
+      if (lenA == c.lenB) {
+        if (wm.s0 + cntA - 1 <= c.s0b) continue;  // no string-1 index above any string-2 index
+        c.diag = true;
+      }
+    }
+    c.mm = a.mmin[lenA * (kLongLen + 1) + c.lenB];
+    if (c.mm > min(lenA, c.lenB)) continue;
+    c.w = max(lenA, c.lenB) / 2 - 1;
+    if constexpr (KIND == kMixed) {
+      const int kind = kind_of_len(c.lenB);
+      if constexpr (LMAX <= 32) {
+        if (kind == kP4)
+          PassDispatch<LMIN, LMAX, kP4, LMAX>::run(lenA, a, c, off);
+        else if (kind == kP3)
+          PassDispatch<LMIN, LMAX, kP3, LMAX>::run(lenA, a, c, off);
+        else if (kind == kP2)
+          PassDispatch<LMIN, LMAX, kP2, LMAX>::run(lenA, a, c, off);
+        else if (kind == kW32)
+          PassDispatch<LMIN, LMAX, kW32, LMAX>::run(lenA, a, c, off);
+        else
+          tile_pass_rolled<kW64, LMAX>(a, c, off, lenA);
+      } else {
+        if (kind == kP4)
+          tile_pass_rolled<kP4, LMAX>(a, c, off, lenA);
+        else if (kind == kP3)
+          tile_pass_rolled<kP3, LMAX>(a, c, off, lenA);
+        else if (kind == kP2)
+          tile_pass_rolled<kP2, LMAX>(a, c, off, lenA);
+        else if (kind == kW32)
+          tile_pass_rolled<kW32, LMAX>(a, c, off, lenA);
+        else
+          tile_pass_rolled<kW64, LMAX>(a, c, off, lenA);
+      }
+    } else if constexpr (LMAX <= 32 && KIND != kW64) {
+      PassDispatch<LMIN, LMAX, KIND, LMAX>::run(lenA, a, c, off);
+    } else {
+      tile_pass_rolled<KIND, LMAX>(a, c, off, lenA);
+    }
+  }
+  if (wave_ok && c.qn > 0) flush_queue(a, c.q, c.qn, (uint64_t)wm.s0);
+}
+
+// LMIN..LMAX: the string-1 lengths this instance handles.
+// `ap` points to the arguments in device memory, so the rarely called
+// exact-scoring functions take them by reference without a private copy.
+template <int LMIN, int LMAX, int KIND>
+__global__ void __launch_bounds__(kThreads) k_jw_tiles(const JwArgs* __restrict__ ap, int64_t wave0, int64_t nwaves,
+                                                      int64_t tile0, int64_t ntiles, int tiles_per_chunk) {
+  extern __shared__ uint32_t lds[];
+  const int64_t t_begin = tile0 + (int64_t)blockIdx.y * tiles_per_chunk;
+  const int64_t t_end = min(t_begin + tiles_per_chunk, tile0 + ntiles);
+  run_item<LMIN, LMAX, KIND>(*ap, lds, wave0 + (int64_t)blockIdx.x * kWaves, wave0 + nwaves, t_begin, t_end);
+}
+
+// Complemented position masks of every string-2 tile: one thread per
+// (tile column, alphabet row).
+__global__ void __launch_bounds__(256) k_build_table(const uint8_t* __restrict__ chars,
+                                                     const int64_t* __restrict__ off,
+                                                     const uint64_t* __restrict__ sorted,
+                                                     const TileMeta* __restrict__ tiles, int64_t ntiles,
+                                                     const uint8_t* __restrict__ amap, int krows, int tw,
+                                                     uint32_t* __restrict__ table) {
+  const int64_t total = ntiles * (int64_t)tw;
+  const int r = blockIdx.y;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = g / tw;
+    const int col = (int)(g - t * tw);
+    const TileMeta tm = tiles[t];
+    uint32_t word = 0xFFFFFFFFu;
+    if (r < krows - 1) {
+      auto mask_of = [&](int sidx) -> uint64_t {
+        if (sidx >= tm.count) return 0ull;
+        const int64_t e = (int64_t)sorted[tm.s0 + sidx];
+        const int64_t o = off[e];
+        uint64_t m = 0;
+        for (int j = 0; j < tm.len; ++j)
+          if (amap[chars[o + j]] == r) m |= 1ull << j;
+        return m;
+      };
+      const int kind = kind_of_len(tm.len);
+      if (packed_kind(kind)) {
+        const int nl = kind_lanes(kind), lb = kind_lane_bits(kind);
+        for (int k = 0; k < nl; ++k) word &= ~((uint32_t)mask_of(nl * col + k) << (1 + k * lb));
+      } else if (kind == kW32) {
+        word = ~(uint32_t)mask_of(col);
+      } else {
+        const uint64_t m = mask_of(col >> 1);
+        word = ~(uint32_t)((col & 1) ? (m >> 32) : m);
+      }
+    }
+    table[((size_t)t * krows + r) * tw + col] = word;
+  }
+}
+
+template <int LMIN, int LMAX, int KIND>
+static int launch_tiles(const JwArgs* args, size_t lds_bytes, int64_t w0, int64_t nw, int64_t t0, int64_t nt,
+                        hipStream_t s) {
+  if (nw <= 0 || nt <= 0) return FH_OK;
+  const int64_t gx = ceil_div(nw, kWaves);
+  int tpc = 16;
+  int64_t gy = ceil_div(nt, tpc);
+  while (gy > 65535) {
+    tpc *= 2;
+    gy = ceil_div(nt, tpc);
+  }
+  FH_REQUIRE(gx < (1ll << 31), "too many string-1 waves");
+  hipLaunchKernelGGL((k_jw_tiles<LMIN, LMAX, KIND>), dim3((unsigned)gx, (unsigned)gy), dim3(kThreads), lds_bytes, s, args,
+                     w0, nw, t0, nt, tpc);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+
+// Dispatch from a runtime kind to the instance of one length range.
+template <int LMIN, int LMAX>
+int launch_kind(int kind, const JwArgs* args, size_t lds, int64_t w0, int64_t nw, int64_t t0, int64_t nt,
+                hipStream_t s) {
+  switch (kind) {
+    case kP4: return launch_tiles<LMIN, LMAX, kP4>(args, lds, w0, nw, t0, nt, s);
+    case kP3: return launch_tiles<LMIN, LMAX, kP3>(args, lds, w0, nw, t0, nt, s);
+    case kP2: return launch_tiles<LMIN, LMAX, kP2>(args, lds, w0, nw, t0, nt, s);
+    case kW32: return launch_tiles<LMIN, LMAX, kW32>(args, lds, w0, nw, t0, nt, s);
+    case kW64: return launch_tiles<LMIN, LMAX, kW64>(args, lds, w0, nw, t0, nt, s);
+    default: set_error("launch_kind: bad kind %d", kind); return FH_ERR_INTERNAL;
+  }
+}
+
+}  // namespace
+}  // namespace fh
