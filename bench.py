@@ -40,6 +40,29 @@ class _HostGroups:
         self.chars, self.offsets, self.n_unique = chars, offsets, n_unique
 
 
+INT8_MFMA_PEAK_OPS = 5.0e15  # dense int8 MFMA, MI355X
+
+
+def comparison_roofline(stats_all, kernel_name, traffic):
+    """Roofline of the comparison kernel from the per-call work accounting:
+    int8 MFMA ops of the bound (bound-filtered form) or algorithmic VALU
+    lane-ops (bit-parallel form), over the HIP-event time of the launches."""
+    import numpy as np
+
+    kern_ms = float(np.mean([sum(s["pass_ms"] for s in st) for st in stats_all]))
+    mfma = float(np.mean([sum(s["alg_mfma_ops"] for s in st) for st in stats_all]))
+    lane = float(np.mean([sum(s["alg_lane_ops"] for s in st) for st in stats_all]))
+    if mfma > 0:
+        achieved = mfma / (kern_ms / 1e3)
+        return {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_MFMA_PEAK_OPS / 1e12,
+                "unit": "TOP/s (int8)", "frac": achieved / INT8_MFMA_PEAK_OPS, "traffic": traffic,
+                "kernel": kernel_name, "kernel_ms": kern_ms, "alg_ops": mfma}
+    achieved = lane / (kern_ms / 1e3)
+    return {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
+            "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK, "traffic": traffic,
+            "kernel": kernel_name, "kernel_ms": kern_ms, "alg_ops": lane}
+
+
 def measured_traffic(workload, n):
     """HBM bytes moved by the comparison-kernel dispatches of one step, from
     the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
@@ -69,6 +92,9 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of 256 MiB three times (calibrates PMC byte counters)")
     ap.add_argument("--n", type=int, default=None, help="records per side (default 1M link, 2M dedup)")
+    ap.add_argument("--method", choices=["tiles", "bound"], default="tiles",
+                    help="all-pairs comparison: the bit-parallel match of every pair (tiles, default), or an int8 "
+                         "MFMA bound with exact scoring of the candidates (bound)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration")
@@ -90,6 +116,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    from faster import _lib
+
+    _lib.set_jw_method(args.method)
     if args.copy_probe:
         src = torch.empty(256 << 20, dtype=torch.uint8, device=dev).random_(0, 255)
         dst = torch.empty_like(src)
@@ -151,10 +180,10 @@ def main():
     comps = float(n) * float(n) * 3
     value = comps / (ms / 1e3)
 
-    # Roofline of the comparison kernel: algorithmic lane-ops / kernel time.
-    kern_ms = np.mean([sum(s["kernel_ms"] for s in st) for st in stats_all])
-    alg_ops = np.mean([sum(s["alg_lane_ops"] for s in st) for st in stats_all])
-    achieved = alg_ops / (kern_ms / 1e3)
+    kname = "k_jw_bound (int8 MFMA bound + exact candidates, 3 vars)" if args.method == "bound" else \
+        "k_jw_tiles (bit-parallel, 3 vars)"
+    roof = comparison_roofline(stats_all, kname, measured_traffic(f"link_{args.method}", n))
+    kern_ms = roof["kernel_ms"]
     st0 = stats_all[-1]
     counts = np.concatenate([[n * n - int(n_union)], np.diff(np.asarray(offs))[1:]]).astype(np.int64)
     level_sizes = [int(k.numel()) for k in keys]
@@ -186,17 +215,16 @@ def main():
         "dtype": "u8",
         "data": "synthetic (seeded generator, lognormal lengths, 30% perturbed copies)",
         "config": {"workload": f"plain linkage {n} x {n} records, 3 string vars + int32 house number, "
-                               "thresholds [0.88, 0.94], p = 0.1", "n_a": n, "n_b": n,
+                               "thresholds [0.88, 0.94], p = 0.1", "n_a": n, "n_b": n, "method": args.method,
                    "string_vars": list(STRING_VARS), "exact_vars": ["house_number"], "patterns": 54},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
-                     "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK,
-                     "traffic": measured_traffic("link", n),
-                     "kernel": "k_jw_tiles (all-pairs JW, 3 vars)", "kernel_ms": kern_ms,
-                     "alg_lane_ops": alg_ops},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "extras": {
             "pairs_kernel_per_var": [s["pairs_kernel"] for s in st0],
             "pairs_pruned_per_var": [s["pairs_pruned"] for s in st0],
+            "candidates_per_var": [s["pairs_candidates"] for s in st0],
+            "comparison_stage_ms_per_var": [s["kernel_ms"] for s in st0],
+            "scoring_ms_per_var": [s["eval_ms"] for s in st0],
             "alphabet": [s["alphabet"] for s in st0],
             "kernel_comparisons_per_s": comps / (kern_ms / 1e3),
             "level_list_sizes": level_sizes,
@@ -261,9 +289,8 @@ def bench_dedup(args):
         log(f"step {k + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms")
     ms = 1e3 * sum(times) / len(times)
     comps = float(n) * float(n - 1) / 2 * 3
-    kern_ms = np.mean([sum(s["kernel_ms"] for s in st) for st in stats_all])
-    alg_ops = np.mean([sum(s["alg_lane_ops"] for s in st) for st in stats_all])
-    achieved = alg_ops / (kern_ms / 1e3)
+    roof = comparison_roofline(stats_all, f"k_jw_{'bound' if args.method == 'bound' else 'tiles'} (triangle, 3 vars)",
+                               measured_traffic(f"dedup_{args.method}", n))
     np.random.seed(0)
     est = faster.Estimation(3, 1, counts)
     with contextlib.redirect_stdout(sys.stderr):  # stdout carries only the JSON line
@@ -274,10 +301,8 @@ def bench_dedup(args):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8", "data": "synthetic (20% perturbed duplicates in clusters of 2-5)",
         "config": {"workload": f"deduplication of {n} records, 3 string vars + int16 birth year (row ids)",
-                   "n": n, "patterns": 54},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
-                     "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK,
-                     "traffic": measured_traffic("dedup", n), "kernel": "k_jw_tiles (triangle, 3 vars)", "kernel_ms": kern_ms, "alg_lane_ops": alg_ops},
+                   "n": n, "patterns": 54, "method": args.method},
+        "roofline": roof,
         "cpu_baseline": None,
         "extras": {"level_list_sizes": [int(k.numel()) for k in keys], "counts_head": counts[:4].tolist(),
                    "exact_equal_pairs": int(patterns.exact_pattern_totals([by.group], [by.group], True)[1]),

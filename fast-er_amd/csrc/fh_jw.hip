@@ -4,10 +4,26 @@
 // deduplication copy faster/deduplication.py:132-196) and its thresholding
 // and compaction (faster/comparison.py:321-338). The comparison kernel and
 // its design are in fh_jw_core.h.
+#include <atomic>
 #include <memory>
 #include <mutex>
 
 #include "fh_jw_core.h"
+
+namespace fh {
+namespace {
+std::atomic<int> g_jw_method{FH_JW_METHOD_TILES};
+}  // namespace
+}  // namespace fh
+
+extern "C" int fh_jw_set_method(int method) {
+  fh::clear_error();
+  FH_REQUIRE(method == FH_JW_METHOD_TILES || method == FH_JW_METHOD_BOUND, "fh_jw_set_method: bad method %d", method);
+  fh::g_jw_method.store(method);
+  return FH_OK;
+}
+
+extern "C" int fh_jw_get_method(void) { return fh::g_jw_method.load(); }
 
 namespace fh {
 namespace {
@@ -105,8 +121,10 @@ struct Plan {
 
 static int lrange_of(int len) { return len <= 16 ? 0 : (len <= 32 ? 1 : 2); }
 
-static int launch_kinds(const JwArgs* args, size_t lds, const Plan& pl, int r, hipStream_t s, int* launches) {
-  const int64_t w0 = pl.wave_lo[r], nw = pl.wave_hi[r] - pl.wave_lo[r];
+// Launches the waves of length range r that lie in [u0, u1) against every tile kind.
+static int launch_kinds(const JwArgs* args, size_t lds, const Plan& pl, int r, int64_t u0, int64_t u1, hipStream_t s,
+                        int* launches) {
+  const int64_t w0 = std::max(pl.wave_lo[r], u0), nw = std::min(pl.wave_hi[r], u1) - w0;
   if (nw <= 0) return FH_OK;
   for (int k = 0; k < kNumKinds; ++k) {
     const int64_t t0 = pl.tile_lo[k], nt = pl.tile_hi[k] - pl.tile_lo[k];
@@ -272,6 +290,20 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     }
     pl.tile_hi[k] = (int64_t)pl.tiles.size();
   }
+  // Bound-filtered form: string-2 tiles of kBTile strings of one length.
+  const int method = g_jw_method.load();
+  const bool bound = method == FH_JW_METHOD_BOUND;
+  std::vector<TileMeta> btiles;
+  if (bound) {
+    for (int c = 1; c <= kLongLen; ++c)
+      for (int64_t s0 = start_b[c]; s0 < start_b[c + 1]; s0 += kBTile) {
+        TileMeta tm;
+        tm.len = c;
+        tm.count = (int32_t)std::min<int64_t>(kBTile, start_b[c + 1] - s0);
+        tm.s0 = s0;
+        btiles.push_back(tm);
+      }
+  }
   // Work accounting (host arithmetic over the length histograms).
   if (stats) {
     stats->alphabet = K;
@@ -298,7 +330,10 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       }
     stats->pairs_kernel = (int64_t)kern;
     stats->pairs_pruned = (int64_t)pruned;
-    stats->alg_lane_ops = ops;
+    stats->alg_lane_ops = bound ? 0.0 : ops;
+    // One 64-byte dot product (64 multiply-adds) per pair.
+    stats->alg_mfma_ops = bound ? kern * 128.0 : 0.0;
+    stats->method = method;
   }
 
   // Upload plan, tables.
@@ -317,6 +352,35 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   uint32_t* d_table = nullptr;
   const int64_t ntiles = (int64_t)pl.tiles.size();
   FH_TRY(scr.alloc(&d_table, (size_t)std::max<int64_t>(1, ntiles) * krows * tw));
+  TileMeta* d_btiles = nullptr;
+  uint64_t* d_cand = nullptr;  // candidate pairs of one round of the bound pass
+  unsigned long long* cand_count = nullptr;
+  const int64_t cand_cap = (int64_t)1 << 27;
+  int64_t* d_cls = nullptr;
+  uint8_t *d_planes_a = nullptr, *d_planes_b = nullptr;
+  const int64_t nbt = (int64_t)btiles.size();
+  if (bound) {
+    FH_TRY(scr.alloc(&d_btiles, std::max<size_t>(1, btiles.size())));
+    FH_TRY(scr.alloc(&d_planes_a, (size_t)std::max<int64_t>(1, n_a) * 64));
+    FH_TRY(scr.alloc(&d_planes_b, (size_t)std::max<int64_t>(1, nbt) * kBTile * 64));
+    if (nbt > 0)
+      FH_HIP(hipMemcpyAsync(d_btiles, btiles.data(), btiles.size() * sizeof(TileMeta), hipMemcpyHostToDevice, s));
+    FH_TRY(jw_bound_build_planes(chars_a, offsets_a, ent_a, n_a, chars_b, offsets_b, ent_b, d_btiles, nbt, d_small, K,
+                                 d_planes_a, d_planes_b, s));
+  }
+  // The bound form's candidate pairs go to one buffer per round and are
+  // scored by a separate kernel, which finds a string-2 value's table column
+  // from its sorted index through the class starts and first tile of each
+  // class.
+  if (bound) {
+    FH_TRY(scr.alloc(&d_cand, (size_t)cand_cap));
+    FH_TRY(scr.alloc(&cand_count, 1));
+    std::vector<int64_t> cls(2 * kNumClasses, 0);
+    for (int c = 0; c < kNumClasses; ++c) cls[c] = start_b[c];
+    for (int64_t t = (int64_t)pl.tiles.size() - 1; t >= 0; --t) cls[kNumClasses + pl.tiles[t].len] = t;
+    FH_TRY(scr.alloc(&d_cls, cls.size()));
+    FH_HIP(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
+  }
   if (ntiles > 0) {
     const int64_t total = ntiles * tw;
     const int64_t bx = std::min<int64_t>(ceil_div(total, 256), 65535);
@@ -324,9 +388,9 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
                        ent_b, d_tiles, ntiles, d_small, krows, tw, d_table);
     FH_HIP(hipGetLastError());
   }
-  unsigned long long* counter = nullptr;
-  FH_TRY(scr.alloc(&counter, 1));
-  FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(counter), 0, 1, s));
+  unsigned long long* counter = nullptr;  // [0] hits, [1] candidates
+  FH_TRY(scr.alloc(&counter, 2));
+  FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(counter), 0, 2, s));
 
   JwArgs args;
   args.chars_a = chars_a;
@@ -356,20 +420,115 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   args.hmask = 0;
   args.code_stride = 0;
   args.overflow = nullptr;
+  args.n_candidates = nullptr;
+  args.cls_start = d_cls;
+  args.cls_tile0 = d_cls ? d_cls + kNumClasses : nullptr;
+  // The bit-parallel form scores its (rare) survivors inside its VALU-bound
+  // kernel; the bound form hands its candidates to the scoring kernel.
+  args.cand = bound ? d_cand : nullptr;
+  args.cand_cap = cand_cap;
+  args.cand_count = cand_count;
+  args.planes_a = d_planes_a;
+  args.planes_b = d_planes_b;
+  if (bound) args.tiles = d_btiles;
 
   JwArgs* d_args = nullptr;
   FH_TRY(scr.alloc(&d_args, 1));
   FH_HIP(hipMemcpyAsync(d_args, &args, sizeof(JwArgs), hipMemcpyHostToDevice, s));
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evr[4] = {nullptr, nullptr, nullptr, nullptr};
   if (stats) {
     FH_HIP(hipEventCreate(&ev0));
     FH_HIP(hipEventCreate(&ev1));
+    for (auto& e : evr) FH_HIP(hipEventCreate(&e));
     FH_HIP(hipEventRecord(ev0, s));
   }
   int launches = 0;
-  FH_TRY(launch_kinds(d_args, lds_bytes, pl, 0, s, &launches));
-  FH_TRY(launch_kinds(d_args, lds_bytes, pl, 1, s, &launches));
-  FH_TRY(launch_kinds(d_args, lds_bytes, pl, 2, s, &launches));
+  int64_t n_candidates = 0;
+  if (!bound) {
+    if (stats) FH_HIP(hipEventRecord(evr[0], s));
+    for (int r = 0; r < 3; ++r)
+      FH_TRY(launch_kinds(d_args, lds_bytes, pl, r, 0, (int64_t)pl.waves.size(), s, &launches));
+    if (stats) {
+      FH_HIP(hipEventRecord(evr[1], s));
+      FH_HIP(hipEventSynchronize(evr[1]));
+      float ms = 0;
+      FH_HIP(hipEventElapsedTime(&ms, evr[0], evr[1]));
+      stats->pass_ms += ms;
+    }
+  } else {
+    // Rounds over string-1 units, each sized so that its candidates fit the
+    // buffer at the candidate rate seen so far; a round that overflows is
+    // split in two and redone (its comparison pass is repeated, nothing is
+    // lost).
+    const int64_t nunits = (int64_t)pl.waves.size();
+    std::vector<double> unit_pairs(nunits + 1, 0.0);  // prefix sums of kernel pairs per unit
+    {
+      std::vector<double> per_len(kLongLen + 1, 0.0);
+      for (int la = 1; la <= kLongLen; ++la)
+        for (int lb = 1; lb <= kLongLen; ++lb) {
+          if (!hb[lb] || mmin[la * (kLongLen + 1) + lb] > std::min(la, lb)) continue;
+          if (!two && lb > la) continue;  // deduplication: string 2 never longer than string 1
+          per_len[la] += (!two && lb == la) ? 0.5 * (double)hb[lb] : (double)hb[lb];
+        }
+      for (int64_t u = 0; u < nunits; ++u)
+        unit_pairs[u + 1] = unit_pairs[u] + (double)pl.waves[u].count * per_len[pl.waves[u].len];
+    }
+    double rate = 4e-3;  // candidates per kernel pair, updated after every round
+    std::vector<std::pair<int64_t, int64_t>> todo;
+    auto next_round = [&](int64_t u0) {
+      const double budget = 0.7 * (double)cand_cap / rate;
+      int64_t u1 = u0 + 1;
+      while (u1 < nunits && unit_pairs[u1 + 1] - unit_pairs[u0] <= budget) ++u1;
+      return u1;
+    };
+    int64_t u0 = 0;
+    while (u0 < nunits || !todo.empty()) {
+      int64_t a0, a1;
+      if (!todo.empty()) {
+        a0 = todo.back().first;
+        a1 = todo.back().second;
+        todo.pop_back();
+      } else {
+        a0 = u0;
+        a1 = next_round(u0);
+        u0 = a1;
+      }
+      FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(cand_count), 0, 1, s));
+      if (stats) FH_HIP(hipEventRecord(evr[0], s));
+      FH_TRY(launch_jw_bound(mode, d_args, a0, a1 - a0, 0, nbt, s));
+      ++launches;
+      if (stats) FH_HIP(hipEventRecord(evr[1], s));
+      uint64_t got = 0;
+      FH_HIP(hipMemcpyAsync(&got, cand_count, sizeof(got), hipMemcpyDeviceToHost, s));
+      FH_HIP(hipStreamSynchronize(s));
+      if (stats) {
+        float ms = 0;
+        FH_HIP(hipEventElapsedTime(&ms, evr[0], evr[1]));
+        stats->pass_ms += ms;
+      }
+      const double pairs = unit_pairs[a1] - unit_pairs[a0];
+      if (pairs > 0) rate = std::max(1e-6, 1.25 * (double)got / pairs);
+      if ((int64_t)got > cand_cap) {
+        FH_REQUIRE(a1 - a0 > 1, "fh_jw_levels: %llu candidates in one string-1 unit exceed the buffer",
+                   (unsigned long long)got);
+        const int64_t mid = a0 + (a1 - a0) / 2;
+        todo.push_back({mid, a1});
+        todo.push_back({a0, mid});
+        continue;
+      }
+      n_candidates += (int64_t)got;
+      if (stats) FH_HIP(hipEventRecord(evr[2], s));
+      FH_TRY(launch_eval_candidates(d_args, (int64_t)got, s));
+      ++launches;
+      if (stats) {
+        FH_HIP(hipEventRecord(evr[3], s));
+        FH_HIP(hipEventSynchronize(evr[3]));
+        float ms = 0;
+        FH_HIP(hipEventElapsedTime(&ms, evr[2], evr[3]));
+        stats->eval_ms += ms;
+      }
+    }
+  }
 
   // Generic pairs: special strings (longer than 64 bytes, and empty strings
   // when a score of 0 reaches level 1).
@@ -409,9 +568,10 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   if (stats) {
     FH_HIP(hipEventRecord(ev1, s));
   }
-  uint64_t h_count = 0;
-  FH_HIP(hipMemcpyAsync(&h_count, counter, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  uint64_t h_count[2] = {0, 0};
+  FH_HIP(hipMemcpyAsync(h_count, counter, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   FH_HIP(hipStreamSynchronize(s));
+  if (stats) stats->pairs_candidates = n_candidates;
   if (stats) {
     float ms = 0;
     FH_HIP(hipEventElapsedTime(&ms, ev0, ev1));
@@ -419,8 +579,9 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     stats->launches = launches;
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
+    for (auto& e : evr) (void)hipEventDestroy(e);
   }
-  *n_hits = (int64_t)h_count;
+  *n_hits = (int64_t)h_count[0];
   return FH_OK;
 }
 

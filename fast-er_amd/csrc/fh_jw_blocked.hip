@@ -464,6 +464,9 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
   FH_TRY(pl->alloc(&pl->n_union, 1));
   for (auto& a : pl->args) {
     a.hkeys = pl->hkeys;
+    a.planes_a = nullptr;
+    a.planes_b = nullptr;
+    a.n_candidates = nullptr;
     a.hcodes = pl->hcodes;
     a.hmask = slots - 1;
     a.overflow = pl->overflow;

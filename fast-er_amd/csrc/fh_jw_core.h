@@ -60,6 +60,7 @@ constexpr int kLongLen = 64;          // longest string on the bit-parallel path
 constexpr int kMaxGenericLen = 1024;  // longest string supported at all
 constexpr int kClassLong = kLongLen + 1;
 constexpr int kNumClasses = kLongLen + 2;  // 0 = empty, 1..64, 65 = long
+constexpr int kBTile = 256;                // string-2 strings per tile of the bound-filtered form
 
 // String-2 mask layouts ("kinds"), chosen by string-2 length.
 enum Kind { kP4 = 0, kP3 = 1, kP2 = 2, kW32 = 3, kW64 = 4 };
@@ -82,6 +83,16 @@ __host__ __device__ constexpr int kind_of_len(int len) {
 __host__ __device__ constexpr int kind_tile_strings(int k, int tw) {
   return packed_kind(k) ? kind_lanes(k) * tw : (k == kW32 ? tw : tw / 2);
 }
+// log2 of the dword columns of a tile of `count` strings that the passes
+// read, rounded up to a power of two >= kU and capped at the tile width.
+__host__ __device__ inline int tile_cols_log2(int kind, int count, int tw) {
+  const int used = packed_kind(kind) ? (count + kind_lanes(kind) - 1) / kind_lanes(kind)
+                                     : (kind == kW32 ? count : 2 * count);
+  int sh = 3;  // kU = 8
+  while ((1 << sh) < used && (1 << sh) < tw) ++sh;
+  return sh;
+}
+
 // Algorithmic VALU lane-ops per pair of string-1 length la: three ops per
 // character and dword (or3, add, select) plus the survivor test (extract and
 // count each packed string, one compare per dword), divided by the strings
@@ -132,6 +143,15 @@ struct JwArgs {
   uint64_t hmask;
   uint32_t code_stride;
   unsigned int* overflow;
+  // Bound-filtered form (fh_jw_bound.hip): per-string count vectors.
+  const uint8_t* planes_a;  // [sorted index][64]
+  const uint8_t* planes_b;  // [tile][4][kBTile][16]
+  unsigned long long* n_candidates;  // pairs scored exactly (may be null)
+  const int64_t* cls_start;          // [kNumClasses] first string-2 sorted index of each length
+  const int64_t* cls_tile0;          // [kNumClasses] first bit-parallel tile of each length
+  uint64_t* cand;                    // candidate buffer of the bound pass
+  int64_t cand_cap;
+  unsigned long long* cand_count;
 };
 
 constexpr unsigned long long kEmptyKey = ~0ull;
@@ -178,6 +198,18 @@ int launch_jw_tiles_r1(int kind, const JwArgs* args, size_t lds_bytes, int64_t w
                        hipStream_t s);
 int launch_jw_tiles_r2(int kind, const JwArgs* args, size_t lds_bytes, int64_t w0, int64_t nw, int64_t t0, int64_t nt,
                        hipStream_t s);
+
+// Bound-filtered comparison (fh_jw_bound.hip): LDS bytes of one workgroup,
+// the per-string count vectors of both sides, and the launch over string-1
+// units [u0, u0 + nu) x string-2 tiles [t0, t0 + nt) of kBTile strings.
+size_t jw_bound_lds_bytes();
+int jw_bound_build_planes(const uint8_t* chars_a, const int64_t* off_a, const uint64_t* sorted_a, int64_t n_a,
+                          const uint8_t* chars_b, const int64_t* off_b, const uint64_t* sorted_b,
+                          const TileMeta* tiles, int64_t ntiles, const uint8_t* amap, int K, uint8_t* planes_a,
+                          uint8_t* planes_b, hipStream_t s);
+int launch_jw_bound(int mode, const JwArgs* args, int64_t u0, int64_t nu, int64_t t0, int64_t nt, hipStream_t s);
+// Exact scoring of the first n entries of args->cand.
+int launch_eval_candidates(const JwArgs* args, int64_t n, hipStream_t s);
 
 // Device code: internal to every translation unit that includes this header.
 namespace {
@@ -289,6 +321,18 @@ __device__ __forceinline__ void hash_add(const JwArgs& a, uint64_t id, uint32_t 
 }
 
 // Evaluates one queued pair exactly and emits its key when its level is > 0.
+// Emits the level key of pair (x = string 1, y = string 2) when its level is > 0.
+__device__ __forceinline__ void emit_level(const JwArgs& a, uint64_t x, uint64_t y, int lvl) {
+  if (lvl <= 0) return;
+  const uint64_t id = (uint64_t)x * (uint64_t)a.n_b + y;
+  if (a.hkeys) {
+    hash_add(a, id, (uint32_t)lvl * a.code_stride);
+  } else {
+    const unsigned long long pos = atomicAdd(a.counter, 1ull);
+    if ((int64_t)pos < a.cap) a.hits[pos] = id * 4ull + (uint64_t)lvl;
+  }
+}
+
 __device__ __noinline__ void eval_pair(const JwArgs& a, uint64_t ea, uint64_t eb) {
   const int64_t oa = a.off_a[ea];
   const int la = (int)(a.off_a[ea + 1] - oa);
@@ -305,20 +349,27 @@ __device__ __noinline__ void eval_pair(const JwArgs& a, uint64_t ea, uint64_t eb
   } else {
     s = jw_exact<1>(sa, la, sb, lb, a.p);
   }
-  const int lvl = level_of(s, a.lo, a.hi);
-  if (lvl > 0) {
-    const uint64_t id = (uint64_t)x * (uint64_t)a.n_b + y;
-    if (a.hkeys) {
-      hash_add(a, id, (uint32_t)lvl * a.code_stride);
-    } else {
-      const unsigned long long pos = atomicAdd(a.counter, 1ull);
-      if ((int64_t)pos < a.cap) a.hits[pos] = id * 4ull + (uint64_t)lvl;
-    }
-  }
+  emit_level(a, x, y, level_of(s, a.lo, a.hi));
 }
 
 // Processes the wave's queue of (string-1 lane, string-2 sorted index) pairs.
+// With a candidate buffer (a.cand), the queue is appended to it as
+// (string-1 sorted index << 32) | string-2 sorted index, with one atomic per
+// flush, for a separate scoring kernel; entries past its capacity are
+// counted only. Without one (the persistent blocked kernel), the pairs are
+// scored here.
 __device__ __noinline__ void flush_queue(const JwArgs& a, const uint64_t* q, int qn, uint64_t s0a) {
+  if (a.cand) {
+    unsigned long long base = 0;
+    if (__lane_id() == 0) base = atomicAdd(a.cand_count, (unsigned long long)qn);
+    base = __shfl(base, 0, 64);
+    for (int k = (int)__lane_id(); k < qn; k += 64) {
+      const uint64_t v = q[k];
+      if (base + (unsigned long long)k < (unsigned long long)a.cand_cap)
+        a.cand[base + k] = ((s0a + (v & 63ull)) << 32) | (v >> 6);
+    }
+    return;
+  }
   for (int k = (int)__lane_id(); k < qn; k += 64) {
     const uint64_t v = q[k];
     const uint64_t ea = a.sorted_a[s0a + (v & 63ull)];
@@ -610,20 +661,24 @@ __device__ __forceinline__ void run_item(const JwArgs& a, uint32_t* lds, int64_t
   }
 
   for (int64_t t = t_begin; t < t_end; ++t) {
+    const TileMeta tm = a.tiles[t];
     __syncthreads();
     {
+      // Only the columns the passes read: the tile's used columns rounded up
+      // to a power of two >= kU (columns past the tile's strings hold stale
+      // data, whose candidates the passes drop by index).
+      const int shift = tile_cols_log2(kind_of_len(tm.len), tm.count, tw);
+      const int cols = 1 << shift;
       const uint32_t* src = a.table + (size_t)t * a.krows * tw;
-      const int total = a.krows * tw;
-      const int shift = __builtin_ctz(tw);
+      const int total = a.krows << shift;
       for (int idx = threadIdx.x; idx < total; idx += kThreads) {
         const int r = idx >> shift;
-        const int col = idx & (tw - 1);
-        lds[r * stride + col] = src[idx];
+        const int col = idx & (cols - 1);
+        lds[r * stride + col] = src[r * tw + col];
       }
     }
     __syncthreads();
     if (!wave_ok) continue;
-    const TileMeta tm = a.tiles[t];
     c.lenB = __builtin_amdgcn_readfirstlane(tm.len);
     c.cntB = __builtin_amdgcn_readfirstlane(tm.count);
     c.s0b = tm.s0;

@@ -2,11 +2,11 @@
 // pair generation, level split, sorted-set membership and the pattern merge.
 //
 // All lists are uint64 "level keys" (pair id * 4 + level). The expansion and
-// exact-pair kernels are load-balanced over their outputs (one thread per
-// output element, which finds its input by binary search over the scanned
-// counts), so a value shared by thousands of rows does not serialise one
-// thread the way the reference's per-input loops do
-// (faster/comparison.py:236-241, faster/deduplication.py:295-332).
+// exact-pair kernels are load-balanced over their outputs (each output finds
+// its input in the scanned counts; see for_each_output), so a value shared by
+// thousands of rows does not serialise one thread the way the reference's
+// per-input loops do (faster/comparison.py:236-241,
+// faster/deduplication.py:295-332).
 #include <algorithm>
 #include <vector>
 
@@ -16,6 +16,8 @@ namespace fh {
 namespace {
 
 constexpr int kThreads = 256;
+
+__device__ __forceinline__ int64_t ceil_div_dev(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Largest index i in [0, n) with offs[i] <= o (offs ascending, offs[0] = 0).
 template <typename T>
@@ -33,6 +35,77 @@ __device__ __forceinline__ int64_t upper_index(const T* __restrict__ offs, int64
 
 inline unsigned grid_for(int64_t n, int64_t cap = 1 << 20) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kThreads), cap));
+}
+
+// Load-balanced expansion over scanned counts: a workgroup owns a tile of
+// kExpTile consecutive outputs. The inputs ("rows") owning them form one
+// contiguous range, found by two binary searches over the scanned offsets;
+// the range's offsets are staged in LDS, so each output searches a few LDS
+// entries instead of the whole offset array.
+constexpr int kExpItems = 8;
+constexpr int kExpTile = kThreads * kExpItems;
+constexpr int kExpLds = 1024;
+
+struct ExpTile {
+  int64_t o0, o1;   // outputs [o0, o1)
+  int64_t lo, cnt;  // owning rows [lo, lo + cnt)
+  bool staged;      // offs[lo .. lo + cnt) copied to LDS
+};
+
+__device__ __forceinline__ ExpTile exp_tile_begin(const uint64_t* __restrict__ offs, int64_t n, int64_t tile,
+                                                  int64_t total, uint64_t* s_offs, int64_t* s_rng) {
+  ExpTile t;
+  t.o0 = tile * (int64_t)kExpTile;
+  t.o1 = min(t.o0 + (int64_t)kExpTile, total);
+  __syncthreads();  // the previous tile's searches are done with s_offs
+  if (threadIdx.x == 0) s_rng[0] = upper_index(offs, n, (uint64_t)t.o0);
+  if (threadIdx.x == 64) s_rng[1] = upper_index(offs, n, (uint64_t)(t.o1 - 1));
+  __syncthreads();
+  t.lo = s_rng[0];
+  t.cnt = s_rng[1] - t.lo + 1;
+  t.staged = t.cnt <= kExpLds;
+  if (t.staged)
+    for (int k = threadIdx.x; k < t.cnt; k += blockDim.x) s_offs[k] = offs[t.lo + k];
+  __syncthreads();
+  return t;
+}
+
+// Row owning output o of the tile: offs[row] <= o < offs[row + 1].
+__device__ __forceinline__ int64_t exp_row(const ExpTile& t, const uint64_t* __restrict__ offs,
+                                           const uint64_t* s_offs, uint64_t o) {
+  if (t.staged) {
+    int64_t lo = 0, hi = t.cnt;  // s_offs[lo] <= o < s_offs[hi] (s_offs[cnt] = +inf)
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (s_offs[mid] <= o)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    return t.lo + lo;
+  }
+  return t.lo + upper_index(offs + t.lo, t.cnt, o);
+}
+
+inline unsigned grid_tiles(int64_t total, int64_t cap = 1 << 18) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, kExpTile), cap));
+}
+
+// Calls f(o, row) for every output o in [0, total), row being its owner.
+template <typename F>
+__device__ __forceinline__ void for_each_output(const uint64_t* __restrict__ offs, int64_t n_rows, int64_t total,
+                                                F&& f) {
+  __shared__ uint64_t s_offs[kExpLds];
+  __shared__ int64_t s_rng[2];
+  const int64_t n_tiles = ceil_div_dev(total, (int64_t)kExpTile);
+  for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const ExpTile t = exp_tile_begin(offs, n_rows, tile, total, s_offs, s_rng);
+    for (int k = 0; k < kExpItems; ++k) {
+      const int64_t o = t.o0 + (int64_t)k * blockDim.x + threadIdx.x;
+      if (o >= t.o1) break;
+      f(o, exp_row(t, offs, s_offs, (uint64_t)o));
+    }
+  }
 }
 
 // --------------------------- linkage expansion -----------------------------
@@ -54,8 +127,7 @@ __global__ void __launch_bounds__(kThreads) k_expand_link(const uint64_t* __rest
                                                           const int64_t* __restrict__ rows_b, int64_t n_b,
                                                           const uint64_t* __restrict__ offs, int64_t total,
                                                           uint64_t* __restrict__ out) {
-  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t h = upper_index(offs, n_hits, (uint64_t)o);
+  for_each_output(offs, n_hits, total, [&](int64_t o, int64_t h) {
     const uint64_t key = hits[h];
     const uint64_t id = key >> 2;
     const uint64_t ua = id / (uint64_t)n_ub, ub = id % (uint64_t)n_ub;
@@ -64,7 +136,7 @@ __global__ void __launch_bounds__(kThreads) k_expand_link(const uint64_t* __rest
     const int64_t ra = rows_a[starts_a[ua] + (int64_t)(k / cb)];
     const int64_t rb = rows_b[starts_b[ub] + (int64_t)(k % cb)];
     out[o] = ((uint64_t)ra * (uint64_t)n_b + (uint64_t)rb) * 4ull + (key & 3ull);
-  }
+  });
 }
 
 // -------------------------- deduplication expansion ------------------------
@@ -113,8 +185,7 @@ __global__ void __launch_bounds__(kThreads) k_expand_dedup(const uint64_t* __res
                                                            const int64_t* __restrict__ rows, int64_t n,
                                                            const uint64_t* __restrict__ offs, int64_t total,
                                                            uint64_t* __restrict__ out) {
-  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t h = upper_index(offs, n_virtual, (uint64_t)o);
+  for_each_output(offs, n_virtual, total, [&](int64_t o, int64_t h) {
     uint64_t ux, uy, lvl;
     dedup_hit(hits, n_hits, n_u, diag_level, h, &ux, &uy, &lvl);
     const uint64_t k = (uint64_t)o - offs[h];
@@ -131,7 +202,7 @@ __global__ void __launch_bounds__(kThreads) k_expand_dedup(const uint64_t* __res
     }
     const uint64_t hi = r1 > r2 ? r1 : r2, lo = r1 > r2 ? r2 : r1;
     out[o] = (hi * (uint64_t)n + lo) * 4ull + lvl;
-  }
+  });
 }
 
 // ------------------------------ exact pairs --------------------------------
@@ -149,12 +220,11 @@ __global__ void __launch_bounds__(kThreads) k_exact_link(const int64_t* __restri
                                                          const int64_t* __restrict__ rows_b, int64_t n_b,
                                                          const uint64_t* __restrict__ offs, int64_t total,
                                                          uint64_t* __restrict__ out) {
-  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = upper_index(offs, n_a, (uint64_t)o);
+  for_each_output(offs, n_a, total, [&](int64_t o, int64_t i) {
     const int64_t g = group_a[i];
     const int64_t j = rows_b[starts_b[g] + (o - (int64_t)offs[i])];
     out[o] = ((uint64_t)i * (uint64_t)n_b + (uint64_t)j) * 4ull + 1ull;
-  }
+  });
 }
 
 // Position of every row inside its value group (rows ascending per group).
@@ -173,11 +243,10 @@ __global__ void __launch_bounds__(kThreads) k_exact_dedup(const int64_t* __restr
                                                           const int64_t* __restrict__ rows,
                                                           const uint64_t* __restrict__ offs, int64_t total,
                                                           uint64_t* __restrict__ out) {
-  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < total; o += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = upper_index(offs, n, (uint64_t)o);
+  for_each_output(offs, n, total, [&](int64_t o, int64_t i) {
     const int64_t j = rows[starts[group[i]] + (o - (int64_t)offs[i])];
     out[o] = ((uint64_t)i * (uint64_t)n + (uint64_t)j) * 4ull + 1ull;
-  }
+  });
 }
 
 // ----------------------------- level split ---------------------------------
@@ -418,7 +487,7 @@ extern "C" int fh_expand_link(const uint64_t* hits, int64_t n_hits, int64_t n_ub
   FH_TRY(scan_total(cnt, n_hits, tot, &total, s));
   *n_out = (int64_t)total;
   if ((int64_t)total > capacity || total == 0) return FH_OK;
-  hipLaunchKernelGGL(k_expand_link, dim3(grid_for((int64_t)total)), dim3(kThreads), 0, s, hits, n_hits, n_ub,
+  hipLaunchKernelGGL(k_expand_link, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, hits, n_hits, n_ub,
                      starts_a, rows_a, starts_b, rows_b, n_b, cnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
   FH_HIP(hipStreamSynchronize(s));
@@ -446,7 +515,7 @@ extern "C" int fh_expand_dedup(const uint64_t* hits, int64_t n_hits, int64_t n_u
   FH_TRY(scan_total(cnt, n_virtual, tot, &total, s));
   *n_out = (int64_t)total;
   if ((int64_t)total > capacity || total == 0) return FH_OK;
-  hipLaunchKernelGGL(k_expand_dedup, dim3(grid_for((int64_t)total)), dim3(kThreads), 0, s, hits, n_hits, n_virtual,
+  hipLaunchKernelGGL(k_expand_dedup, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, hits, n_hits, n_virtual,
                      n_u, diag_level, starts, rows, n, cnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
   FH_HIP(hipStreamSynchronize(s));
@@ -473,7 +542,7 @@ extern "C" int fh_exact_link(const int64_t* group_a, int64_t n_a, const int64_t*
   FH_TRY(scan_total(cnt, n_a, tot, &total, s));
   *n_out = (int64_t)total;
   if ((int64_t)total > capacity || total == 0) return FH_OK;
-  hipLaunchKernelGGL(k_exact_link, dim3(grid_for((int64_t)total)), dim3(kThreads), 0, s, group_a, n_a, starts_b,
+  hipLaunchKernelGGL(k_exact_link, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, group_a, n_a, starts_b,
                      rows_b, n_b, cnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
   FH_HIP(hipStreamSynchronize(s));
@@ -499,7 +568,7 @@ extern "C" int fh_exact_dedup(const int64_t* group, int64_t n, const int64_t* st
   FH_TRY(scan_total(cnt, n, tot, &total, s));
   *n_out = (int64_t)total;
   if ((int64_t)total > capacity || total == 0) return FH_OK;
-  hipLaunchKernelGGL(k_exact_dedup, dim3(grid_for((int64_t)total)), dim3(kThreads), 0, s, group, n, starts, rows,
+  hipLaunchKernelGGL(k_exact_dedup, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, group, n, starts, rows,
                      cnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
   FH_HIP(hipStreamSynchronize(s));

@@ -40,6 +40,12 @@ class JwStats(ctypes.Structure):
         ("alphabet", ctypes.c_int32),
         ("launches", ctypes.c_int32),
         ("kernel_ms", ctypes.c_double),
+        ("alg_mfma_ops", ctypes.c_double),
+        ("method", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("pairs_candidates", ctypes.c_int64),
+        ("pass_ms", ctypes.c_double),
+        ("eval_ms", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -49,6 +55,8 @@ class JwStats(ctypes.Structure):
 _SIGS = {
     "fh_last_error": (ctypes.c_char_p, []),
     "fh_version": (_i32, []),
+    "fh_jw_set_method": (_i32, [_i32]),
+    "fh_jw_get_method": (_i32, []),
     "fh_jw_levels": (_i32, [_vp, _vp, _i64, _vp, _vp, _i64, _i32, _f32, _f32, _f32, _vp, _i64,
                             ctypes.POINTER(_i64), ctypes.POINTER(JwStats), _vp]),
     "fh_expand_link": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
@@ -145,3 +153,23 @@ def ptr(t: torch.Tensor | None):
 
 def device() -> torch.device:
     return torch.device("cuda", torch.cuda.current_device())
+
+
+FH_JW_METHOD_TILES = 0
+FH_JW_METHOD_BOUND = 1
+_METHODS = {"tiles": FH_JW_METHOD_TILES, "bound": FH_JW_METHOD_BOUND}
+
+
+def set_jw_method(name: str) -> None:
+    """Selects the all-pairs comparison algorithm of every later call
+    (process-wide): "tiles" (bit-parallel greedy match of every pair; the
+    default) or "bound" (int8 MFMA bound, then exact scoring of the
+    candidates). Both give identical results."""
+    if name not in _METHODS:
+        raise ValueError(f"unknown comparison method {name!r}; expected one of {sorted(_METHODS)}")
+    check(load().fh_jw_set_method(_METHODS[name]), "fh_jw_set_method")
+
+
+def jw_method() -> str:
+    m = load().fh_jw_get_method()
+    return {v: k for k, v in _METHODS.items()}[m]

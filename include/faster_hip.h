@@ -54,14 +54,37 @@ int fh_version(void);
 /* Work accounting of one fh_jw_levels call (host struct, optional). */
 typedef struct {
   int64_t pairs_logical;   /* value pairs the call covers */
-  int64_t pairs_kernel;    /* pairs evaluated by the bit-parallel kernel */
+  int64_t pairs_kernel;    /* pairs evaluated by the comparison kernel (not pruned by length) */
   int64_t pairs_pruned;    /* pairs skipped because their lengths cannot reach level 1 */
   int64_t pairs_generic;   /* pairs evaluated by the generic kernel (empty or > 64 bytes) */
   double alg_lane_ops;     /* algorithmic VALU lane-ops of the kernel pairs */
   int32_t alphabet;        /* byte values present on both sides */
   int32_t launches;        /* kernel launches of the comparison stage */
   double kernel_ms;        /* device time of the comparison kernels (HIP events) */
+  double alg_mfma_ops;     /* bound-filtered form: int8 MFMA ops of the bound (128 per kernel pair) */
+  int32_t method;          /* FH_JW_METHOD_* used */
+  int32_t reserved;
+  int64_t pairs_candidates; /* pairs handed to the scoring kernel (bound passed / greedy count passed) */
+  double pass_ms;          /* device time of the all-pairs pass kernels alone (HIP events) */
+  double eval_ms;          /* device time of the candidate scoring kernels (HIP events) */
 } fh_jw_stats;
+
+/*
+ * Comparison algorithm of fh_jw_levels (process-wide; both give identical
+ * results):
+ *   FH_JW_METHOD_TILES (default): the bit-parallel greedy match of every pair
+ *     (VALU).
+ *   FH_JW_METHOD_BOUND: an int8 MFMA pass bounds every pair's match count by
+ *     its symbol-count overlap; pairs whose bound reaches the pruning
+ *     threshold are scored exactly.
+ * No counterpart in the reference (one CUDA kernel scores every pair,
+ * faster/comparison.py:133-186); the switch exists to time and check the two
+ * forms against each other.
+ */
+#define FH_JW_METHOD_TILES 0
+#define FH_JW_METHOD_BOUND 1
+int fh_jw_set_method(int method);
+int fh_jw_get_method(void);
 
 /*
  * All-pairs Jaro-Winkler levels over two sets of byte strings.

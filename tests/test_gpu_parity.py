@@ -110,6 +110,24 @@ def test_exact_link(oracle):
     _eq_lists(exact_gpu(Sa, Sb), oracle.exact(Sa, Sb), "exact str")
 
 
+def test_exact_link_load_balance(oracle):
+    """Expansion tiles that span one huge group (3,000 B rows per A row), and
+    tiles whose owners are spread over thousands of empty rows (beyond the
+    LDS-staged row window), against the oracle."""
+    from faster.comparison import exact_gpu
+    from faster.deduplication import exact_dedup_gpu
+
+    rng = np.random.default_rng(21)
+    A = np.concatenate([rng.integers(10 ** 6, 2 * 10 ** 6, 19000), np.full(1000, 7)])
+    rng.shuffle(A)
+    B = np.concatenate([np.full(3000, 7), rng.integers(0, 100, 2000), rng.integers(10 ** 6, 2 * 10 ** 6, 3000)])
+    rng.shuffle(B)
+    _eq_lists(exact_gpu(A, B), oracle.exact(A, B), "exact load balance")
+    E = np.concatenate([rng.integers(10 ** 6, 2 * 10 ** 6, 30000), np.full(900, 3)])
+    rng.shuffle(E)
+    _eq_lists(exact_dedup_gpu(E), oracle.exact_dedup(E), "exact dedup load balance")
+
+
 def test_sort_and_scan_primitives():
     import faster._ops as ops
 
@@ -292,3 +310,47 @@ def test_blocked_linkage_matches_per_block_loop(oracle):
     assert res
     bl.fit()
     np.testing.assert_array_equal(bl.Block_Counts, ref)
+
+
+@pytest.mark.parametrize("mode", ["link", "dedup"])
+def test_comparison_methods_agree(mode):
+    """The MFMA bound-filtered form and the bit-parallel form return the same
+    level keys on synthetic data with every length class (1..64 bytes, every
+    mask kind) and many near-duplicates."""
+    from faster import _lib, _ops
+    from faster.synthetic import random_strings
+
+    rng = np.random.default_rng(31)
+    A = random_strings(rng, 3000, 8.0, 0.6, 1, 64)
+    B = random_strings(rng, 2500, 8.0, 0.6, 1, 64)
+    for k in range(0, 2500, 3):
+        s = A[(k * 7) % 3000]
+        B[k] = s[:-1] + "x" if len(s) > 1 else s
+    dev = _lib.device()
+    out = {}
+    default = _lib.jw_method()
+    for method in ("tiles", "bound"):
+        _lib.set_jw_method(method)
+        try:
+            if mode == "link":
+                ca, oa = _pack(A, dev)
+                cb, ob = _pack(B, dev)
+                hits, st = _ops.jw_levels(ca, oa, len(A), cb, ob, len(B), _lib.FH_MODE_LINK, 0.1, 0.88, 0.94,
+                                          want_stats=True)
+            else:
+                ca, oa = _pack(A + B, dev)
+                hits, st = _ops.jw_levels(ca, oa, len(A) + len(B), ca, oa, len(A) + len(B), _lib.FH_MODE_DEDUP, 0.1,
+                                          0.88, 0.94, want_stats=True)
+        finally:
+            _lib.set_jw_method(default)
+        assert st["method"] == _lib._METHODS[method]
+        out[method] = np.sort(hits.cpu().numpy())
+    assert out["tiles"].size > 500
+    np.testing.assert_array_equal(out["bound"], out["tiles"])
+
+
+def _pack(strings, dev):
+    from faster.encode import pack_strings
+
+    c, o = pack_strings(strings)
+    return torch.from_numpy(c if c.size else np.zeros(1, np.uint8)).to(dev), torch.from_numpy(o).to(dev)
