@@ -381,13 +381,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     FH_TRY(scr.alloc(&d_cls, cls.size()));
     FH_HIP(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
   }
-  if (ntiles > 0) {
-    const int64_t total = ntiles * tw;
-    const int64_t bx = std::min<int64_t>(ceil_div(total, 256), 65535);
-    hipLaunchKernelGGL(k_build_table, dim3((unsigned)bx, (unsigned)krows), dim3(256), 0, s, chars_b, offsets_b,
-                       ent_b, d_tiles, ntiles, d_small, krows, tw, d_table);
-    FH_HIP(hipGetLastError());
-  }
+  FH_TRY(launch_build_table(chars_b, offsets_b, ent_b, d_tiles, ntiles, d_small, krows, tw, d_table, s));
   unsigned long long* counter = nullptr;  // [0] hits, [1] candidates
   FH_TRY(scr.alloc(&counter, 2));
   FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(counter), 0, 2, s));

@@ -179,12 +179,8 @@ struct BlockedPlan {
     FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(hist), 0, (int64_t)(n_patterns * (uint64_t)n_blocks), stream));
     FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(n_union), 0, 1, stream));
     for (int v = 0; v < n_vars; ++v) {
-      if (ntiles[v] == 0) continue;
-      const int64_t total = ntiles[v] * tw;
-      const int64_t bx = std::min<int64_t>(ceil_div(total, 256), 65535);
-      hipLaunchKernelGGL(k_build_table, dim3((unsigned)bx, (unsigned)krows[v]), dim3(256), 0, stream, chars_b[v],
-                         off_b[v], sorted_b[v], tiles_dev[v], ntiles[v], amap_dev[v], krows[v], tw, table[v]);
-      FH_HIP(hipGetLastError());
+      FH_TRY(launch_build_table(chars_b[v], off_b[v], sorted_b[v], tiles_dev[v], ntiles[v], amap_dev[v], krows[v], tw,
+                                table[v], stream));
     }
     if (n_items[0] > 0) {
       hipLaunchKernelGGL((k_jw_blocked<1, 32>), dim3((unsigned)grid), dim3(kThreads), lds_bytes, stream, args_dev,
@@ -273,6 +269,7 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
   std::vector<uint8_t> mmin((kLongLen + 1) * (kLongLen + 1), 255);
   std::vector<bool> mm_done((kLongLen + 1) * (kLongLen + 1), false);
   std::vector<WorkItem> items[2];
+  std::vector<int32_t> item_keys[2];  // tile kind * 256 + string-1 length of each item
   int max_krows = 1;
   pl->args.resize(n_fuzzy);
 
@@ -374,7 +371,10 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
     wave_lo[n_blocks] = (int64_t)waves.size();
     tile_lo[n_blocks] = (int64_t)tiles.size();
     // Work items: per block, groups of waves (short: 1..32 bytes; long:
-    // 33..64) x chunks of tiles; cost ~ lanes x strings x length.
+    // 33..64) x chunks of tiles; cost ~ lanes x strings x length. Items are
+    // later ordered by (first tile kind, first string-1 length), so
+    // co-resident workgroups mostly run the same unrolled pass, then
+    // largest first.
     for (int32_t b = 0; b < n_blocks; ++b) {
       const int64_t t0 = tile_lo[b], t1 = tile_lo[b + 1];
       if (t1 == t0) continue;
@@ -396,6 +396,7 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
             it.ntiles = std::min<int64_t>(tu.tiles_per_item, t1 - t);
             it.cost = lanes * tile_cost * (double)it.ntiles / (double)(t1 - t0);
             items[part].push_back(it);
+            item_keys[part].push_back(kind_of_len(tiles[t].len) * 256 + waves[w].len);
           }
         }
       }
@@ -475,8 +476,17 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
   FH_TRY(pl->alloc(&pl->args_dev, pl->args.size()));
   FH_HIP(hipMemcpyAsync(pl->args_dev, pl->args.data(), pl->args.size() * sizeof(JwArgs), hipMemcpyHostToDevice, s));
   for (int part = 0; part < 2; ++part) {
-    std::stable_sort(items[part].begin(), items[part].end(),
-                     [](const WorkItem& x, const WorkItem& y) { return x.cost > y.cost; });
+    std::vector<size_t> order(items[part].size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+    const auto& keys = item_keys[part];
+    const auto& its = items[part];
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+      if (keys[x] != keys[y]) return keys[x] < keys[y];
+      return its[x].cost > its[y].cost;
+    });
+    std::vector<WorkItem> sorted_items(order.size());
+    for (size_t k = 0; k < order.size(); ++k) sorted_items[k] = its[order[k]];
+    items[part].swap(sorted_items);
     pl->n_items[part] = (int64_t)items[part].size();
     FH_TRY(pl->alloc(&pl->items_dev[part], std::max<size_t>(1, items[part].size())));
     if (!items[part].empty())

@@ -739,44 +739,61 @@ __global__ void __launch_bounds__(kThreads) k_jw_tiles(const JwArgs* __restrict_
   run_item<LMIN, LMAX, KIND>(*ap, lds, wave0 + (int64_t)blockIdx.x * kWaves, wave0 + nwaves, t_begin, t_end);
 }
 
-// Complemented position masks of every string-2 tile: one thread per
-// (tile column, alphabet row).
+// Complemented position masks of every string-2 tile, built in LDS: one
+// workgroup per tile sets its krows x tw words to all ones, clears one bit
+// per string byte with an LDS atomic (bit j of string s in the lane, column
+// and kind its length gives), and writes the tile out with coalesced stores.
+// Rows of bytes absent from string 1's side and columns past the tile's
+// strings stay all ones. LDS: krows * tw * 4 bytes.
 __global__ void __launch_bounds__(256) k_build_table(const uint8_t* __restrict__ chars,
                                                      const int64_t* __restrict__ off,
                                                      const uint64_t* __restrict__ sorted,
                                                      const TileMeta* __restrict__ tiles, int64_t ntiles,
                                                      const uint8_t* __restrict__ amap, int krows, int tw,
                                                      uint32_t* __restrict__ table) {
-  const int64_t total = ntiles * (int64_t)tw;
-  const int r = blockIdx.y;
-  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t t = g / tw;
-    const int col = (int)(g - t * tw);
+  extern __shared__ uint32_t tl[];
+  const int total = krows * tw;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const TileMeta tm = tiles[t];
-    uint32_t word = 0xFFFFFFFFu;
-    if (r < krows - 1) {
-      auto mask_of = [&](int sidx) -> uint64_t {
-        if (sidx >= tm.count) return 0ull;
-        const int64_t e = (int64_t)sorted[tm.s0 + sidx];
-        const int64_t o = off[e];
-        uint64_t m = 0;
-        for (int j = 0; j < tm.len; ++j)
-          if (amap[chars[o + j]] == r) m |= 1ull << j;
-        return m;
-      };
-      const int kind = kind_of_len(tm.len);
+    __syncthreads();
+    for (int i = threadIdx.x; i < total; i += blockDim.x) tl[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    const int kind = kind_of_len(tm.len);
+    const int n = tm.count * tm.len;
+    for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+      const int sidx = idx / tm.len;
+      const int j = idx - sidx * tm.len;
+      const int64_t e = (int64_t)sorted[tm.s0 + sidx];
+      const int code = amap[chars[off[e] + j]];
+      if (code >= krows - 1) continue;
+      int col, bit;
       if (packed_kind(kind)) {
-        const int nl = kind_lanes(kind), lb = kind_lane_bits(kind);
-        for (int k = 0; k < nl; ++k) word &= ~((uint32_t)mask_of(nl * col + k) << (1 + k * lb));
+        col = sidx / kind_lanes(kind);
+        bit = 1 + (sidx % kind_lanes(kind)) * kind_lane_bits(kind) + j;
       } else if (kind == kW32) {
-        word = ~(uint32_t)mask_of(col);
+        col = sidx;
+        bit = j;
       } else {
-        const uint64_t m = mask_of(col >> 1);
-        word = ~(uint32_t)((col & 1) ? (m >> 32) : m);
+        col = 2 * sidx + (j >> 5);
+        bit = j & 31;
       }
+      atomicAnd(&tl[code * tw + col], ~(1u << bit));
     }
-    table[((size_t)t * krows + r) * tw + col] = word;
+    __syncthreads();
+    uint32_t* dst = table + (size_t)t * total;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) dst[i] = tl[i];
   }
+}
+
+// Launch of k_build_table over tiles [0, ntiles).
+inline int launch_build_table(const uint8_t* chars, const int64_t* off, const uint64_t* sorted, const TileMeta* tiles,
+                              int64_t ntiles, const uint8_t* amap, int krows, int tw, uint32_t* table, hipStream_t s) {
+  if (ntiles <= 0) return FH_OK;
+  const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 1 << 16);
+  hipLaunchKernelGGL(k_build_table, dim3(grid), dim3(256), (size_t)krows * tw * 4, s, chars, off, sorted, tiles,
+                     ntiles, amap, krows, tw, table);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
 }
 
 template <int LMIN, int LMAX, int KIND>
