@@ -449,16 +449,25 @@ __device__ __forceinline__ int lane_count(uint32_t na, int k) {
   return __popc((na >> (k * LB)) & ((1u << LB) - 1u));
 }
 
+// Bits [0, n) of a dword, n clipped to [0, 32].
+__device__ __forceinline__ uint32_t low_bits(int64_t n) {
+  return n <= 0 ? 0u : (n >= 32 ? ~0u : ((1u << n) - 1u));
+}
+
 // Queues the candidates whose bit is set in `smask` (bit k = string-2 tile
-// index first + k), after the validity and triangle checks. Rare path.
+// index first + k, k < n), after the validity and triangle checks. Rare
+// path: each lane walks its own set bits, so the wave loops as often as the
+// busiest lane has candidates (almost always once).
 __device__ __forceinline__ void push_candidates(const JwArgs& a, PassCtx& c, uint32_t smask, int first, int n) {
-  const int lane = __lane_id();
+  uint32_t m = c.lane_ok ? smask & low_bits(min(n, c.cntB - first)) : 0u;
+  // Triangle: string-2 index s0b + first + k below string-1 index s0a + lane.
+  if (c.diag) m &= low_bits(c.s0a + (int64_t)__lane_id() - c.s0b - first);
 #pragma unroll 1
-  for (int k = 0; k < n; ++k) {
-    const int sidx = first + k;
-    bool pred = c.lane_ok && ((smask >> k) & 1u) && sidx < c.cntB;
-    if (c.diag) pred = pred && (c.s0a + lane > c.s0b + sidx);
-    queue_push(a, c.q, c.qn, pred, (uint64_t)(c.s0b + sidx), (uint64_t)c.s0a);
+  while (__ballot(m != 0u) != 0) {
+    const bool pred = m != 0u;
+    const int k = pred ? __builtin_ctz(m) : 0;
+    queue_push(a, c.q, c.qn, pred, (uint64_t)(c.s0b + first + k), (uint64_t)c.s0a);
+    m &= m - 1u;
   }
 }
 
@@ -517,13 +526,15 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
         for (int k = 0; k < NL; ++k) mx = max(mx, (uint32_t)lane_count<KIND>(na[u], k));
       }
       if (__ballot((int)mx >= mm && c.lane_ok) == 0) continue;
-      uint32_t smask = 0;
+      // Bit NL * u + k set when pair (u, k) falls short: the sign of m - mm.
+      uint32_t fail = 0;
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
 #pragma unroll
-        for (int k = 0; k < NL; ++k) smask |= (uint32_t)(lane_count<KIND>(na[u], k) >= mm) << (NL * u + k);
+        for (int k = 0; k < NL; ++k)
+          fail |= ((uint32_t)(lane_count<KIND>(na[u], k) - mm) >> 31) << (NL * u + k);
       }
-      push_candidates(a, c, smask, NL * col, NL * kU);
+      push_candidates(a, c, ~fail, NL * col, NL * kU);
     }
   } else {
     uint32_t nwin[LA];
