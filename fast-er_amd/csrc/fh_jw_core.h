@@ -54,7 +54,22 @@ namespace fh {
 
 constexpr int kWaves = 8;             // waves per workgroup
 constexpr int kThreads = kWaves * 64;
-constexpr int kU = 8;                 // dword columns per inner step
+#ifndef FH_JW_KU
+#define FH_JW_KU 8
+#endif
+// Waves per SIMD the comparison kernel is compiled for. A workgroup is
+// 8 waves (2 per SIMD), so only even values change residency; 6 (3 workgroups
+// per CU, 80 VGPRs) measured fastest on every workload, ahead of 4 (the
+// compiler's own choice) and 8 (heavy spills).
+#ifndef FH_JW_WAVES_PER_EU
+#define FH_JW_WAVES_PER_EU 6
+#endif
+constexpr int kU = FH_JW_KU;          // dword columns per inner step (4 or 8)
+constexpr int kULog2 = kU == 8 ? 3 : 2;
+static_assert(kU == 4 || kU == 8, "kU must be 4 or 8");
+#ifndef FH_JW_PREFETCH
+#define FH_JW_PREFETCH 1
+#endif
 constexpr int kQCap = 128;            // survivor queue entries per wave
 constexpr int kLongLen = 64;          // longest string on the bit-parallel path
 constexpr int kMaxGenericLen = 1024;  // longest string supported at all
@@ -88,7 +103,7 @@ __host__ __device__ constexpr int kind_tile_strings(int k, int tw) {
 __host__ __device__ inline int tile_cols_log2(int kind, int count, int tw) {
   const int used = packed_kind(kind) ? (count + kind_lanes(kind) - 1) / kind_lanes(kind)
                                      : (kind == kW32 ? count : 2 * count);
-  int sh = 3;  // kU = 8
+  int sh = kULog2;
   while ((1 << sh) < used && (1 << sh) < tw) ++sh;
   return sh;
 }
@@ -416,6 +431,41 @@ __device__ __forceinline__ const uint32_t* row_ptr(const uint32_t* lds, uint32_t
   return reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(lds) + off_bytes) + col;
 }
 
+// Row of string-1 character I at byte column colb of the staged tile. The
+// row byte offsets are packed two per dword (a workgroup's LDS is at most
+// 64 KB), and one SDWA add reads the 16-bit half in place: as plain shifts
+// the compiler would unpack every offset into a register of its own before
+// the column loop, which costs the occupancy this halving is for.
+// colb is the LDS address of the column (lds_addr of the staged tile plus
+// 4 * col), so the sum is the row's LDS address itself. The callers' loops
+// over i are fully unrolled, so i is a constant here.
+typedef __attribute__((address_space(3))) const uint32_t lds_u32;
+__device__ __forceinline__ uint32_t lds_addr(const uint32_t* p) {
+  return (uint32_t)(uintptr_t)(lds_u32*)p;
+}
+template <int N>
+__device__ __forceinline__ lds_u32* lds_row(const uint32_t (&o)[N], int i, uint32_t colb) {
+  uint32_t r;
+  if (i & 1)
+    asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_1 src1_sel:DWORD"
+        : "=v"(r)
+        : "v"(o[i >> 1]), "v"(colb));
+  else
+    asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_0 src1_sel:DWORD"
+        : "=v"(r)
+        : "v"(o[i >> 1]), "v"(colb));
+  return (lds_u32*)(uintptr_t)r;
+}
+// The same for a run-time i (rolled passes): a select chain, no indexing.
+template <int N>
+__device__ __forceinline__ uint32_t row_off_rt(const uint32_t (&o)[N], int i) {
+  uint32_t w = 0;
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+    if (k == (i >> 1)) w = o[k];
+  return (i & 1) ? (w >> 16) : (w & 0xFFFFu);
+}
+
 // Window of string-2 positions [i - w, i + w] clipped to [0, lenB).
 __device__ __forceinline__ uint64_t window_bits(int i, int w, int lenB) {
   const int lo_i = max(0, i - w);
@@ -475,7 +525,7 @@ __device__ __forceinline__ void push_candidates(const JwArgs& a, PassCtx& c, uin
 // masks, fully unrolled over the LA characters so the row offsets stay in
 // fixed VGPRs and the windows in SGPRs.
 template <int LA, int KIND, int LMAX>
-__device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uint32_t (&off)[LMAX]) {
+__device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uint32_t (&off)[(LMAX + 1) / 2]) {
   const int lenB = c.lenB;
   if constexpr (packed_kind(KIND)) {
     constexpr int NL = kind_lanes(KIND);
@@ -489,20 +539,26 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
     const int mm = c.mm;
     const int ncols = (c.cntB + NL - 1) / NL;
     for (int col = 0; col < ncols; col += kU) {
+      const uint32_t colb = lds_addr(c.lds) + (uint32_t)col * 4u;
       uint32_t na[kU], cur[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
         na[u] = init;
-        cur[u] = row_ptr(c.lds, off[0], col)[u];
+        cur[u] = lds_row(off, 0, colb)[u];
       }
 #pragma unroll
       for (int i = 0; i < LA; ++i) {
-        // Rows of character i + 1 are read while character i is processed
-        // (the scheduling fences keep the reads ahead of this step's ops).
+        // With FH_JW_PREFETCH, rows of character i + 1 are read while
+        // character i is processed (the scheduling fences keep the reads
+        // ahead of this step's ops); without, other waves hide the latency.
         uint32_t nxt[kU];
-        if (i + 1 < LA) {
+        if (FH_JW_PREFETCH && i + 1 < LA) {
 #pragma unroll
-          for (int u = 0; u < kU; ++u) nxt[u] = row_ptr(c.lds, off[i + 1], col)[u];
+          for (int u = 0; u < kU; ++u) nxt[u] = lds_row(off, i + 1, colb)[u];
+        }
+        if (!FH_JW_PREFETCH && i > 0) {
+#pragma unroll
+          for (int u = 0; u < kU; ++u) cur[u] = lds_row(off, i, colb)[u];
         }
         __builtin_amdgcn_sched_barrier(0);
         uint32_t nc[kU], t[kU];
@@ -513,7 +569,7 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
 #pragma unroll
         for (int u = 0; u < kU; ++u) na[u] = select_bits(nc[u], na[u], t[u]);
         __builtin_amdgcn_sched_barrier(0);
-        if (i + 1 < LA) {
+        if (FH_JW_PREFETCH && i + 1 < LA) {
 #pragma unroll
           for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
         }
@@ -526,7 +582,8 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
         for (int k = 0; k < NL; ++k) mx = max(mx, (uint32_t)lane_count<KIND>(na[u], k));
       }
       if (__ballot((int)mx >= mm && c.lane_ok) == 0) continue;
-      // Bit NL * u + k set when pair (u, k) falls short: the sign of m - mm.
+      // Rare path. Bit NL * u + k set when pair (u, k) falls short: the sign
+      // of m - mm (the counts above stay live for it).
       uint32_t fail = 0;
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
@@ -543,12 +600,13 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
     const uint32_t init = 0u;
     const int mm = c.mm;
     for (int col = 0; col < c.cntB; col += kU) {
+      const uint32_t colb = lds_addr(c.lds) + (uint32_t)col * 4u;
       uint32_t na[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u) na[u] = init;
 #pragma unroll
       for (int i = 0; i < LA; ++i) {
-        const uint32_t* row = row_ptr(c.lds, off[i], col);
+        lds_u32* row = lds_row(off, i, colb);
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
           const uint32_t nc = or3(row[u], nwin[i], na[u]);
@@ -567,7 +625,7 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
 // Any string-1 length (rolled loop over its characters) against any mask
 // width; used for the rare long strings (33..64 bytes on either side).
 template <int KIND, int LMAX>
-__device__ __forceinline__ void tile_pass_rolled(const JwArgs& a, PassCtx& c, const uint32_t (&off)[LMAX], int lenA) {
+__device__ __forceinline__ void tile_pass_rolled(const JwArgs& a, PassCtx& c, const uint32_t (&off)[(LMAX + 1) / 2], int lenA) {
   const int lenB = c.lenB;
   constexpr int per = packed_kind(KIND) ? kind_lanes(KIND) : 1;  // strings per dword
   constexpr int dws = KIND == kW64 ? 2 : 1;                      // dwords per string
@@ -577,11 +635,7 @@ __device__ __forceinline__ void tile_pass_rolled(const JwArgs& a, PassCtx& c, co
     uint64_t na = 0;  // lane counts = m (see tile_pass)
 #pragma unroll 1
     for (int i = 0; i < lenA; ++i) {
-      uint32_t oi = 0;
-#pragma unroll
-      for (int k = 0; k < LMAX; ++k)
-        if (k == i) oi = off[k];
-      const uint32_t* row = row_ptr(c.lds, oi, col);
+      const uint32_t* row = row_ptr(c.lds, row_off_rt(off, i), col);
       if constexpr (packed_kind(KIND)) {
         const uint32_t nc = row[0] | packed_nwin<KIND>(i, c.w, lenB) | (uint32_t)na;
         na = select_bits(nc, (uint32_t)na, nc + kind_ones(KIND));
@@ -611,7 +665,7 @@ __device__ __forceinline__ void tile_pass_rolled(const JwArgs& a, PassCtx& c, co
 // Uniform dispatch from the wave's string-1 length to its unrolled pass.
 template <int LO, int HI, int KIND, int LMAX>
 struct PassDispatch {
-  __device__ static __forceinline__ void run(int lenA, const JwArgs& a, PassCtx& c, const uint32_t (&off)[LMAX]) {
+  __device__ static __forceinline__ void run(int lenA, const JwArgs& a, PassCtx& c, const uint32_t (&off)[(LMAX + 1) / 2]) {
     if (lenA == LO)
       tile_pass<LO, KIND, LMAX>(a, c, off);
     else
@@ -620,7 +674,7 @@ struct PassDispatch {
 };
 template <int HI, int KIND, int LMAX>
 struct PassDispatch<HI, HI, KIND, LMAX> {
-  __device__ static __forceinline__ void run(int lenA, const JwArgs& a, PassCtx& c, const uint32_t (&off)[LMAX]) {
+  __device__ static __forceinline__ void run(int lenA, const JwArgs& a, PassCtx& c, const uint32_t (&off)[(LMAX + 1) / 2]) {
     if (lenA == HI) tile_pass<HI, KIND, LMAX>(a, c, off);
   }
 };
@@ -657,17 +711,19 @@ __device__ __forceinline__ void run_item(const JwArgs& a, uint32_t* lds, int64_t
   c.lane_ok = lane < cntA;
   c.s0a = wm.s0;
 
-  // Byte offsets of each character's LDS row.
-  uint32_t off[LMAX];
+  // Byte offsets of each character's LDS row, two per dword (lds_row).
+  uint32_t off[(LMAX + 1) / 2];
   {
     int64_t base = 0;
     if (c.lane_ok) base = a.off_a[a.sorted_a[wm.s0 + lane]];
     const uint32_t none = (uint32_t)(a.krows - 1);
 #pragma unroll
+    for (int k = 0; k < (LMAX + 1) / 2; ++k) off[k] = 0;
+#pragma unroll
     for (int i = 0; i < LMAX; ++i) {
       uint32_t code = none;
       if (c.lane_ok && i < lenA) code = a.amap[a.chars_a[base + i]];
-      off[i] = code * (uint32_t)stride * 4u;
+      off[i >> 1] |= (code * (uint32_t)stride * 4u) << (16 * (i & 1));
     }
   }
 
@@ -742,7 +798,7 @@ __device__ __forceinline__ void run_item(const JwArgs& a, uint32_t* lds, int64_t
 // `ap` points to the arguments in device memory, so the rarely called
 // exact-scoring functions take them by reference without a private copy.
 template <int LMIN, int LMAX, int KIND>
-__global__ void __launch_bounds__(kThreads) k_jw_tiles(const JwArgs* __restrict__ ap, int64_t wave0, int64_t nwaves,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(FH_JW_WAVES_PER_EU))) k_jw_tiles(const JwArgs* __restrict__ ap, int64_t wave0, int64_t nwaves,
                                                       int64_t tile0, int64_t ntiles, int tiles_per_chunk) {
   extern __shared__ uint32_t lds[];
   const int64_t t_begin = tile0 + (int64_t)blockIdx.y * tiles_per_chunk;
