@@ -25,7 +25,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "fast-er_amd"))
 
 # MI355X peaks used for the roofline fractions.
-VALU_LANE_OPS_PEAK = 256 * 4 * 16 * 2.4e9  # 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 3.93e13 lane-ops/s
+# VALU: the published FP32 vector rate, 157.3 TFLOP/s = 7.86e13 FMA lane-ops/s
+# (256 CUs x 4 SIMD x 32 lanes x 2.4 GHz: each SIMD can issue two VALU
+# instructions per quad-cycle, counted by SQ_ACTIVE_INST_VALU2). A probe of
+# the match recurrence's own ops (bitop3, add) sustains 6.7e13 on the device.
+VALU_LANE_OPS_PEAK = 256 * 4 * 32 * 2.4e9
 HBM_PEAK_GBS = 8000.0
 
 
@@ -63,6 +67,32 @@ def comparison_roofline(stats_all, kernel_name, traffic):
             "kernel": kernel_name, "kernel_ms": kern_ms, "alg_ops": lane}
 
 
+def valu_probe(dev):
+    """Times fh_probe_valu (the match recurrence's three VALU ops on register
+    operands, 2048 x 256 threads) with HIP events on its launch stream and
+    prints the sustained lane-op rate against VALU_LANE_OPS_PEAK."""
+    import ctypes
+
+    import torch
+    from faster import _lib
+
+    L = _lib.load()
+    blocks, iters = 2048, 20000
+    out = torch.empty(blocks * 256, dtype=torch.int32, device=dev)
+    ops = ctypes.c_double(0.0)
+    rates = []
+    for _ in range(4):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _lib.check(L.fh_probe_valu(_lib.ptr(out), blocks, iters, ctypes.byref(ops), _lib.stream()), "fh_probe_valu")
+        e1.record()
+        torch.cuda.synchronize()
+        rates.append(ops.value / (e0.elapsed_time(e1) / 1e3))
+    best = max(rates[1:])
+    print(json.dumps({"valu_probe_lane_ops_per_s": best, "peak": VALU_LANE_OPS_PEAK, "frac": best / VALU_LANE_OPS_PEAK,
+                      "lane_ops_per_launch": ops.value, "rates": rates}), flush=True)
+
+
 def measured_traffic(workload, n):
     """HBM bytes moved by the comparison-kernel dispatches of one step, from
     the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded in
@@ -92,6 +122,8 @@ def main():
     ap.add_argument("--no-loops", action="store_true", help="blocked: skip the per-block loop baselines")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of 256 MiB three times (calibrates PMC byte counters)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="run only the VALU throughput probe of the match recurrence (checks the VALU peak)")
     ap.add_argument("--n", type=int, default=None, help="records per side (default 1M link, 2M dedup)")
     ap.add_argument("--method", choices=["tiles", "bound"], default="tiles",
                     help="all-pairs comparison: the bit-parallel match of every pair (tiles, default), or an int8 "
@@ -128,6 +160,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_per_launch": 256 << 20, "launches": 3}), flush=True)
         return
+    if args.valu_probe:
+        return valu_probe(dev)
     if args.workload == "dedup":
         return bench_dedup(args)
     if args.workload == "blocked":

@@ -55,6 +55,7 @@ class JwStats(ctypes.Structure):
 _SIGS = {
     "fh_last_error": (ctypes.c_char_p, []),
     "fh_version": (_i32, []),
+    "fh_probe_valu": (_i32, [_vp, _i32, _i32, ctypes.POINTER(ctypes.c_double), _vp]),
     "fh_jw_set_method": (_i32, [_i32]),
     "fh_jw_get_method": (_i32, []),
     "fh_jw_levels": (_i32, [_vp, _vp, _i64, _vp, _vp, _i64, _i32, _f32, _f32, _f32, _vp, _i64,

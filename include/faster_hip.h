@@ -51,6 +51,15 @@ const char* fh_last_error(void);
 /* ABI version (major * 10000 + minor * 100 + patch). */
 int fh_version(void);
 
+/* Measurement helper, no counterpart in the reference: launches `blocks` x 256
+ * threads that run `iters` rounds of the comparison kernel's match recurrence
+ * (three VALU ops per dword on register operands, 8 independent dwords per
+ * thread) on `stream`, and reports the lane-ops launched. `out` holds
+ * blocks * 256 uint32 (one per thread, written so no work is dead). The
+ * caller times the launch; bench.py --valu-probe does, to check the VALU
+ * peak its roofline uses. */
+int fh_probe_valu(uint32_t* out, int32_t blocks, int32_t iters, double* lane_ops, fh_stream_t stream);
+
 /* Work accounting of one fh_jw_levels call (host struct, optional). */
 typedef struct {
   int64_t pairs_logical;   /* value pairs the call covers */
