@@ -374,7 +374,7 @@ def bench_blocked(args):
     log(f"encoded in {t_enc:.1f} s; autotuning")
     t0 = time.time()
     best, tune_res = bl.autotune(space=dict(tile_width=[128, 256], tiles_per_item=[2, 8], waves_per_item=[4, 8],
-                                            wgs_per_cu=[1, 2]))
+                                            wgs_per_cu=[2, 3]))
     t_tune = time.time() - t0
     log(f"autotune {t_tune:.1f} s: best {best}")
     for w in range(args.warmup):

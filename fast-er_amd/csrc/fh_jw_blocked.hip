@@ -35,7 +35,7 @@ struct WorkItem {
 };
 
 template <int LMIN, int LMAX>
-__global__ void __launch_bounds__(kThreads) k_jw_blocked(const JwArgs* __restrict__ vars,
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(FH_JW_WAVES_PER_EU))) k_jw_blocked(const JwArgs* __restrict__ vars,
                                                         const WorkItem* __restrict__ items, int64_t n_items,
                                                         unsigned int* head) {
   extern __shared__ uint32_t lds[];

@@ -225,7 +225,7 @@ class BlockedLinkage:
             self.plan(_TUNE_CACHE[sig])
             return self.tuning, {}
         space = space or dict(tile_width=[128, 256], tiles_per_item=[2, 8], waves_per_item=[4, 8],
-                              wgs_per_cu=[1, 2, 4])
+                              wgs_per_cu=[1, 2, 3])
         keys = list(space)
         results = {}
         best, best_ms = None, float("inf")

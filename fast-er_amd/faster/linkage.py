@@ -28,15 +28,23 @@ class Linkage():
         self.Indices = Indices
         self.Ksi = Ksi
 
-    def transform(self, Threshold=0.85):
+    def transform(self, Threshold=0.85, Compat=True):
         """DataFrame of all pairs whose match probability is at least
-        ``Threshold`` (faster/linkage.py:26-72). As in the reference, pattern
-        ``k`` reads ``Indices[k - 1]``, so pattern 0 maps to the last list."""
+        ``Threshold`` (faster/linkage.py:26-72).
+
+        ``Indices`` omits pattern 0 (every variable at level 0), so pattern
+        ``k >= 1`` is ``Indices[k - 1]``. With ``Compat=True`` (the default)
+        pattern 0 reads ``Indices[-1]`` as the reference's does
+        (faster/linkage.py:54), i.e. the last pattern's list is linked when
+        pattern 0's probability reaches the threshold; ``Compat=False`` skips
+        pattern 0 instead (its pairs are never listed)."""
         df_A = self.df_A.add_suffix("_A")
         df_B = self.df_B.add_suffix("_B")
         df_A["Index_A"] = range(len(df_A))
         df_B["Index_B"] = range(len(df_B))
         above = np.ravel(np.argwhere(np.asarray(self.Ksi) >= Threshold))
+        if not Compat:
+            above = above[above > 0]
         if np.sum([_numel(self.Indices[i - 1]) for i in above]) == 0:
             raise Exception("No pair of observations has a conditional match probability exceeding the threshold.")
         dev = _lib.device()

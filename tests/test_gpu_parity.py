@@ -242,6 +242,30 @@ def test_pipeline_linkage_and_posteriors(oracle):
     np.testing.assert_array_equal(post, ref)
 
 
+def test_linkage_pattern_zero_compat():
+    """Pattern 0 above the threshold: the reference links Indices[-1] (the
+    last pattern's pairs, faster/linkage.py:54); Compat=False skips it."""
+    import faster
+
+    dfa, dfb, _ = _frames(300, 320, 6)
+    vf = ["first_name", "last_name"]
+    comp = faster.Comparison(dfa, dfb, vf, vf)
+    comp.fit()
+    last = _np(comp.Indices[-1])
+    assert last.size > 0
+    ksi = np.zeros(len(comp.Indices) + 1)
+    ksi[0] = 1.0
+    out = faster.Linkage(dfa, dfb, comp.Indices, ksi).transform(0.85)
+    np.testing.assert_array_equal(out["Index_A"].to_numpy(), last // len(dfb))
+    np.testing.assert_array_equal(out["Index_B"].to_numpy(), last % len(dfb))
+    with pytest.raises(Exception, match="No pair of observations"):
+        faster.Linkage(dfa, dfb, comp.Indices, ksi).transform(0.85, Compat=False)
+    ksi[1] = 1.0
+    out = faster.Linkage(dfa, dfb, comp.Indices, ksi).transform(0.85, Compat=False)
+    first = _np(comp.Indices[0])
+    np.testing.assert_array_equal(out["Index_B"].to_numpy(), first % len(dfb))
+
+
 @pytest.mark.parametrize("which", ["dedup", "link"])
 def test_rowid_exact_mode(oracle, which):
     """Exact variables kept as row ids: Counts equal the oracle's (listed)
