@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) k_eval_candidates(const JwArgs* __restric
     int col = p, shift = 0;
     if (packed_kind(kind)) {
       col = p / kind_lanes(kind);
-      shift = (p % kind_lanes(kind)) * kind_lane_bits(kind) + 1;
+      shift = kind_lane_shift(kind, p % kind_lanes(kind));
     } else if (kind == kW64) {
       col = 2 * p;
     }

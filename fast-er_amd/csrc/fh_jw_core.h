@@ -77,23 +77,41 @@ constexpr int kClassLong = kLongLen + 1;
 constexpr int kNumClasses = kLongLen + 2;  // 0 = empty, 1..64, 65 = long
 constexpr int kBTile = 256;                // string-2 strings per tile of the bound-filtered form
 
-// String-2 mask layouts ("kinds"), chosen by string-2 length.
+// String-2 mask layouts ("kinds"), chosen by string-2 length. The packed
+// kinds hold L strings per dword in lanes of S bits: lane l's positions
+// start at bit kind_lane_base + l * S, and the bit below them is the lane's
+// guard (always set in nc; it absorbs a carry out of the lane below and is
+// where the lane's +1 enters). Lane 0 of kP3 has no guard: nothing carries
+// into it, and its +1 enters at position 0 itself, which lets three strings
+// of 10 bytes share a dword.
+//   kind  lengths  L  S   guard of lane 0
+//   kP4   1-7      4  8   yes
+//   kP3   8-10     3  11  no
+//   kP2   11-15    2  16  yes
 enum Kind { kP4 = 0, kP3 = 1, kP2 = 2, kW32 = 3, kW64 = 4 };
 constexpr int kNumKinds = 5;
 
 __host__ __device__ constexpr bool packed_kind(int k) { return k <= kP2; }
-// Strings per dword of a packed kind and the width of its lanes.
+// Strings per dword of a packed kind, the stride of its lanes and the bit of
+// lane 0's first position.
 __host__ __device__ constexpr int kind_lanes(int k) { return k == kP4 ? 4 : (k == kP3 ? 3 : 2); }
-__host__ __device__ constexpr int kind_lane_bits(int k) { return k == kP4 ? 8 : (k == kP3 ? 10 : 16); }
-// Bit 0 of every lane (the guard, and the place the per-lane +1 enters).
+__host__ __device__ constexpr int kind_lane_bits(int k) { return k == kP4 ? 8 : (k == kP3 ? 11 : 16); }
+__host__ __device__ constexpr int kind_lane_base(int k) { return k == kP3 ? 0 : 1; }
+// First bit of string position 0 in lane l.
+__host__ __device__ constexpr int kind_lane_shift(int k, int l) { return kind_lane_base(k) + l * kind_lane_bits(k); }
+// The bit each lane's +1 enters at: its guard, or position 0 of a lane with none.
 __host__ __device__ constexpr uint32_t kind_ones(int k) {
-  return k == kP4 ? 0x01010101u : (k == kP3 ? 0x00100401u : 0x00010001u);
+  uint32_t v = 0;
+  for (int l = 0; l < kind_lanes(k); ++l) v |= 1u << (kind_lane_shift(k, l) > 0 ? kind_lane_shift(k, l) - 1 : 0);
+  return v;
 }
-// Bits above the last lane (kP3 only): permanently blocked.
-__host__ __device__ constexpr uint32_t kind_top(int k) { return k == kP3 ? 0xC0000000u : 0u; }
 __host__ __device__ constexpr int kind_of_len(int len) {
-  return len <= 7 ? kP4 : (len <= 9 ? kP3 : (len <= 15 ? kP2 : (len <= 32 ? kW32 : kW64)));
+  return len <= 7 ? kP4 : (len <= 10 ? kP3 : (len <= 15 ? kP2 : (len <= 32 ? kW32 : kW64)));
 }
+// Every lane's positions fit the dword, and the lanes fill it.
+static_assert(kind_ones(kP4) == 0x01010101u && kind_lane_shift(kP4, 3) + 7 == 32, "kP4 layout");
+static_assert(kind_ones(kP3) == 0x00200401u && kind_lane_shift(kP3, 2) + 10 == 32, "kP3 layout");
+static_assert(kind_ones(kP2) == 0x00010001u && kind_lane_shift(kP2, 1) + 15 == 32, "kP2 layout");
 // Strings of one tile of `tw` dword columns.
 __host__ __device__ constexpr int kind_tile_strings(int k, int tw) {
   return packed_kind(k) ? kind_lanes(k) * tw : (k == kW32 ? tw : tw / 2);
@@ -476,27 +494,26 @@ __device__ __forceinline__ uint64_t window_bits(int i, int w, int lenB) {
 }
 
 // Complemented window of string-1 character i in every lane of a packed
-// kind: lane bits 1..lenB are the positions, the guard (bit 0) and the bits
-// past the last lane are set.
+// kind: the lanes' positions inside the window are clear; the guards, the
+// positions outside the window or past lenB and the bits above the last lane
+// are set.
 template <int KIND>
 __device__ __forceinline__ uint32_t packed_nwin(int i, int w, int lenB) {
-  constexpr int LB = kind_lane_bits(KIND);
-  constexpr uint32_t lane_mask = (1u << LB) - 1u;
-  const uint32_t lane = (~((uint32_t)window_bits(i, w, lenB) << 1)) & lane_mask;
-  uint32_t v = kind_top(KIND);
+  const uint32_t win = (uint32_t)window_bits(i, w, lenB);
+  uint32_t v = 0;
 #pragma unroll
-  for (int k = 0; k < kind_lanes(KIND); ++k) v |= lane << (k * LB);
-  return v;
+  for (int l = 0; l < kind_lanes(KIND); ++l) v |= win << kind_lane_shift(KIND, l);
+  return ~v;
 }
 
-// Match count of lane k of a packed dword. The guards and the bits above the
+// Match count of lane l of a packed dword. The guards and the bits above the
 // last lane are never set in `na` (they are set in nc, so the select keeps
 // them), so the last lane needs no mask.
 template <int KIND>
-__device__ __forceinline__ int lane_count(uint32_t na, int k) {
+__device__ __forceinline__ int lane_count(uint32_t na, int l) {
   constexpr int LB = kind_lane_bits(KIND);
-  if (k == kind_lanes(KIND) - 1) return __popc(na >> (k * LB));
-  return __popc((na >> (k * LB)) & ((1u << LB) - 1u));
+  if (l == kind_lanes(KIND) - 1) return __popc(na >> kind_lane_shift(KIND, l));
+  return __popc((na >> kind_lane_shift(KIND, l)) & ((1u << (LB - 1)) - 1u));
 }
 
 // Bits [0, n) of a dword, n clipped to [0, 32].
@@ -530,6 +547,8 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
   if constexpr (packed_kind(KIND)) {
     constexpr int NL = kind_lanes(KIND);
     constexpr uint32_t ONES = kind_ones(KIND);
+    constexpr int KU = kU;
+    static_assert(NL * KU <= 32, "one survivor mask bit per pair of a step");
     uint32_t nwin[LA];
 #pragma unroll
     for (int i = 0; i < LA; ++i) nwin[i] = __builtin_amdgcn_readfirstlane(packed_nwin<KIND>(i, c.w, lenB));
@@ -538,11 +557,11 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
     const uint32_t init = 0u;
     const int mm = c.mm;
     const int ncols = (c.cntB + NL - 1) / NL;
-    for (int col = 0; col < ncols; col += kU) {
+    for (int col = 0; col < ncols; col += KU) {
       const uint32_t colb = lds_addr(c.lds) + (uint32_t)col * 4u;
-      uint32_t na[kU], cur[kU];
+      uint32_t na[KU], cur[KU];
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
+      for (int u = 0; u < KU; ++u) {
         na[u] = init;
         cur[u] = lds_row(off, 0, colb)[u];
       }
@@ -551,33 +570,33 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
         // With FH_JW_PREFETCH, rows of character i + 1 are read while
         // character i is processed (the scheduling fences keep the reads
         // ahead of this step's ops); without, other waves hide the latency.
-        uint32_t nxt[kU];
+        uint32_t nxt[KU];
         if (FH_JW_PREFETCH && i + 1 < LA) {
 #pragma unroll
-          for (int u = 0; u < kU; ++u) nxt[u] = lds_row(off, i + 1, colb)[u];
+          for (int u = 0; u < KU; ++u) nxt[u] = lds_row(off, i + 1, colb)[u];
         }
         if (!FH_JW_PREFETCH && i > 0) {
 #pragma unroll
-          for (int u = 0; u < kU; ++u) cur[u] = lds_row(off, i, colb)[u];
+          for (int u = 0; u < KU; ++u) cur[u] = lds_row(off, i, colb)[u];
         }
         __builtin_amdgcn_sched_barrier(0);
-        uint32_t nc[kU], t[kU];
+        uint32_t nc[KU], t[KU];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) nc[u] = or3(cur[u], nwin[i], na[u]);
+        for (int u = 0; u < KU; ++u) nc[u] = or3(cur[u], nwin[i], na[u]);
 #pragma unroll
-        for (int u = 0; u < kU; ++u) t[u] = nc[u] + ONES;
+        for (int u = 0; u < KU; ++u) t[u] = nc[u] + ONES;
 #pragma unroll
-        for (int u = 0; u < kU; ++u) na[u] = select_bits(nc[u], na[u], t[u]);
+        for (int u = 0; u < KU; ++u) na[u] = select_bits(nc[u], na[u], t[u]);
         __builtin_amdgcn_sched_barrier(0);
         if (FH_JW_PREFETCH && i + 1 < LA) {
 #pragma unroll
-          for (int u = 0; u < kU; ++u) cur[u] = nxt[u];
+          for (int u = 0; u < KU; ++u) cur[u] = nxt[u];
         }
       }
-      // Largest match count over the NL * kU pairs.
+      // Largest match count over the NL * KU pairs.
       uint32_t mx = 0;
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
+      for (int u = 0; u < KU; ++u) {
 #pragma unroll
         for (int k = 0; k < NL; ++k) mx = max(mx, (uint32_t)lane_count<KIND>(na[u], k));
       }
@@ -586,12 +605,12 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
       // of m - mm (the counts above stay live for it).
       uint32_t fail = 0;
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
+      for (int u = 0; u < KU; ++u) {
 #pragma unroll
         for (int k = 0; k < NL; ++k)
           fail |= ((uint32_t)(lane_count<KIND>(na[u], k) - mm) >> 31) << (NL * u + k);
       }
-      push_candidates(a, c, ~fail, NL * col, NL * kU);
+      push_candidates(a, c, ~fail, NL * col, NL * KU);
     }
   } else {
     uint32_t nwin[LA];
@@ -836,7 +855,7 @@ __global__ void __launch_bounds__(256) k_build_table(const uint8_t* __restrict__
       int col, bit;
       if (packed_kind(kind)) {
         col = sidx / kind_lanes(kind);
-        bit = 1 + (sidx % kind_lanes(kind)) * kind_lane_bits(kind) + j;
+        bit = kind_lane_shift(kind, sidx % kind_lanes(kind)) + j;
       } else if (kind == kW32) {
         col = sidx;
         bit = j;

@@ -1,9 +1,9 @@
 """The device algorithms' bit-level steps, restated on the CPU and checked
 against independent references (no GPU):
 
-* the guard-bit lane recurrence of the comparison kernel (several string-2
-  masks packed in one 32-bit word, one plain add per character) gives, per
-  lane, the match count of a maximum bipartite matching of the Jaro window
+* the guard-bit lane recurrence of the comparison kernel (2 to 4 string-2
+  masks packed in one 32-bit word, one plain add per character, lane 0
+  without a guard in the 3-lane layout) gives, per lane, the match count of a maximum bipartite matching of the Jaro window
   graph (scipy) -- the count the reference's greedy loop yields
   (faster/comparison.py:48-64, pinned in test_oracle.py);
 * the table-based scorer of the candidate kernel (greedy match from
@@ -21,8 +21,18 @@ import pytest
 
 from test_oracle import _max_matching
 
-LANES = {4: 8, 3: 10, 2: 16}  # strings per word -> lane width (bit 0 = guard)
-MAXLEN = {4: 7, 3: 9, 2: 15}
+# Packed layouts of the comparison kernel: strings per dword -> (lane stride
+# S in bits, bit of lane 0's first position). Lane l's positions start at
+# base + l * S; the bit below is the lane's guard (none under lane 0 when
+# base is 0). Strings of up to S - 1 bytes fit a lane.
+LAYOUTS = {4: (8, 1), 3: (11, 0), 2: (16, 1)}
+
+
+def _layout(nl):
+    stride, base = LAYOUTS[nl]
+    shifts = [base + l * stride for l in range(nl)]
+    ones = sum(1 << max(sh - 1, 0) for sh in shifts)
+    return stride, shifts, ones
 
 
 def _window(i, w, lb):
@@ -32,11 +42,9 @@ def _window(i, w, lb):
 
 def _packed_counts(s1, group, nl):
     """Lane counts of the packed recurrence: nc = mask | window | used,
-    t = nc + ONES, used = nc ? used : t (bitwise), complemented masks."""
-    lb_bits = LANES[nl]
-    lane_mask = (1 << lb_bits) - 1
-    ones = sum(1 << (k * lb_bits) for k in range(nl))
-    top = (0xFFFFFFFF >> (nl * lb_bits)) << (nl * lb_bits) & 0xFFFFFFFF if nl * lb_bits < 32 else 0
+    t = nc + ONES, used = nc ? used : t (bitwise), complemented masks, one
+    32-bit word for all lanes."""
+    stride, shifts, ones = _layout(nl)
     lb = len(group[0])
     w = max(len(s1), lb) // 2 - 1
     na = 0
@@ -44,22 +52,27 @@ def _packed_counts(s1, group, nl):
         word = 0xFFFFFFFF
         for k, s2 in enumerate(group):
             m = sum(1 << j for j, ch in enumerate(s2) if ch == c)
-            word &= ~(m << (1 + k * lb_bits)) & 0xFFFFFFFF
-        lane_win = (~(_window(i, w, lb) << 1)) & lane_mask
-        nwin = top
-        for k in range(nl):
-            nwin |= lane_win << (k * lb_bits)
+            word &= ~(m << shifts[k]) & 0xFFFFFFFF
+        nwin = ~sum(_window(i, w, lb) << sh for sh in shifts) & 0xFFFFFFFF
         nc = word | nwin | na
         t = (nc + ones) & 0xFFFFFFFF
         na = ((nc & na) | (~nc & t)) & 0xFFFFFFFF
-    return [bin((na >> (k * lb_bits)) & lane_mask).count("1") for k in range(nl)]
+    return [bin((na >> sh) & ((1 << (stride - 1)) - 1)).count("1") for sh in shifts]
+
+
+def test_layouts_fill_one_dword():
+    for nl, (stride, base) in LAYOUTS.items():
+        _, shifts, _ = _layout(nl)
+        assert shifts[-1] + stride - 1 <= 32, nl  # last lane's positions fit
+        assert base == 1 or nl * (stride - 1) + (nl - 1) == 32, nl  # guardless lane 0 only when needed
 
 
 @pytest.mark.parametrize("nl", [4, 3, 2])
 def test_guard_lane_recurrence_equals_maximum_matching(nl):
     rng = np.random.default_rng(40 + nl)
+    maxlen = LAYOUTS[nl][0] - 1
     for _ in range(1500):
-        lb = int(rng.integers(1, MAXLEN[nl] + 1))
+        lb = int(rng.integers(1, maxlen + 1))
         la = int(rng.integers(1, 33))
         alpha = ["ab", "abc", "abcdefghijklmnopqrstuvwxyz"][int(rng.integers(0, 3))]
         s1 = "".join(alpha[k] for k in rng.integers(0, len(alpha), la))
