@@ -601,16 +601,21 @@ __device__ __forceinline__ void tile_pass(const JwArgs& a, PassCtx& c, const uin
         for (int k = 0; k < NL; ++k) mx = max(mx, (uint32_t)lane_count<KIND>(na[u], k));
       }
       if (__ballot((int)mx >= mm && c.lane_ok) == 0) continue;
-      // Rare path. Bit NL * u + k set when pair (u, k) falls short: the sign
-      // of m - mm (the counts above stay live for it).
-      uint32_t fail = 0;
+      // Rare path (the counts above stay live for it). Bit NL * u + k set
+      // when pair (u, k) reaches mm; only the dwords in which some lane has
+      // such a pair (usually one) build their bits.
+      uint32_t pass = 0;
 #pragma unroll
       for (int u = 0; u < KU; ++u) {
+        uint32_t mu = 0;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) mu = max(mu, (uint32_t)lane_count<KIND>(na[u], k));
+        if (__ballot((int)mu >= mm && c.lane_ok) == 0) continue;
 #pragma unroll
         for (int k = 0; k < NL; ++k)
-          fail |= ((uint32_t)(lane_count<KIND>(na[u], k) - mm) >> 31) << (NL * u + k);
+          pass |= (uint32_t)(lane_count<KIND>(na[u], k) >= mm) << (NL * u + k);
       }
-      push_candidates(a, c, ~fail, NL * col, NL * KU);
+      push_candidates(a, c, pass, NL * col, NL * KU);
     }
   } else {
     uint32_t nwin[LA];
