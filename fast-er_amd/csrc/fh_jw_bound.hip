@@ -358,7 +358,10 @@ int launch_eval_candidates(const JwArgs* args, int64_t n, hipStream_t s) {
 int launch_jw_bound(int mode, const JwArgs* args, int64_t u0, int64_t nu, int64_t t0, int64_t nt, hipStream_t s) {
   if (nu <= 0 || nt <= 0) return FH_OK;
   const int64_t gx = ceil_div(nu, (int64_t)kBWaves * kUnitsPerWave);
+  // Fewer tiles per workgroup when there are few string-1 units, so that
+  // the launch still fills the GPU.
   int tpc = 16;
+  while (tpc > 1 && gx * ceil_div(nt, tpc) < 2048) tpc /= 2;
   int64_t gy = ceil_div(nt, tpc);
   while (gy > 65535) {
     tpc *= 2;

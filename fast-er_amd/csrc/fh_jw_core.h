@@ -892,7 +892,11 @@ static int launch_tiles(const JwArgs* args, size_t lds_bytes, int64_t w0, int64_
                         hipStream_t s) {
   if (nw <= 0 || nt <= 0) return FH_OK;
   const int64_t gx = ceil_div(nw, kWaves);
+  // 16 tiles per workgroup, fewer when there are few string-1 waves (the
+  // lengths with few strings): a launch of a handful of workgroups would run
+  // each one's tiles one after another on an otherwise idle GPU.
   int tpc = 16;
+  while (tpc > 1 && gx * ceil_div(nt, tpc) < 2048) tpc /= 2;
   int64_t gy = ceil_div(nt, tpc);
   while (gy > 65535) {
     tpc *= 2;
