@@ -321,9 +321,32 @@ __device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* a, int64_t lo
   return lo;
 }
 
+// lower_bound over sorted a[lo, hi) by the 32 lanes of one half-wave (`half`
+// = 0 or 1, all 32 lanes call it with the same arguments): each round probes
+// 32 evenly spaced positions with one load each and keeps the interval
+// between the last probe below x and the first at or above it, so a search
+// of n entries takes about log32(n) dependent loads instead of log2(n).
+__device__ int64_t lower_bound_u64_half(const uint64_t* a, int64_t lo, int64_t hi, uint64_t x, int half) {
+  const int j = (int)(__lane_id() & 31);
+  const uint64_t half_mask = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
+  while (hi - lo > 32) {
+    const int64_t step = (hi - lo + 31) / 32;
+    const int64_t p = lo + step * j;
+    const bool below = p < hi && a[p] < x;
+    const int k = __popcll(__ballot(below) & half_mask);  // probes below x: positions 0 .. k-1
+    const int64_t nlo = k > 0 ? lo + step * (k - 1) + 1 : lo;
+    const int64_t nhi = min(hi, lo + step * k);
+    lo = nlo;
+    hi = nhi;
+  }
+  const int64_t p = lo + j;
+  const bool below = p < hi && a[p] < x;
+  return lo + __popcll(__ballot(below) & half_mask);
+}
+
 __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, uint32_t* __restrict__ codes) {
   __shared__ uint64_t seg[kMergeLds];
-  __shared__ int64_t range[2];
+  __shared__ int64_t ranges[kMaxVars][2];
   const uint64_t* src = m.lists[v];
   const int64_t n = m.sizes[v];
   const int64_t t0 = (int64_t)blockIdx.x * kMergeTile;
@@ -342,15 +365,24 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
   }
   const uint64_t id_lo = src[t0] >> 2;
   const uint64_t id_hi = src[t1 - 1] >> 2;
+  // The ranges of this tile's ids in every other list, all at once: wave q
+  // takes lists q, q + 4, ..., its lower half-wave the first key of the
+  // range and its upper half-wave the end.
+  {
+    const int wave = threadIdx.x >> 6;
+    const int half = (threadIdx.x >> 5) & 1;
+    for (int w = wave; w < m.n_vars; w += kThreads / 64) {
+      if (w == v) continue;
+      const uint64_t x = half ? id_hi * 4ull + 4ull : id_lo * 4ull;
+      const int64_t b = lower_bound_u64_half(m.lists[w], 0, m.sizes[w], x, half);
+      if ((threadIdx.x & 31) == 0) ranges[w][half] = b;
+    }
+  }
+  __syncthreads();
   for (int w = 0; w < m.n_vars; ++w) {
     if (w == v) continue;
     const uint64_t* tl = m.lists[w];
-    if (threadIdx.x == 0) {
-      range[0] = lower_bound_u64(tl, 0, m.sizes[w], id_lo * 4ull);
-      range[1] = lower_bound_u64(tl, range[0], m.sizes[w], id_hi * 4ull + 4ull);
-    }
-    __syncthreads();
-    const int64_t b0 = range[0], b1 = range[1];
+    const int64_t b0 = ranges[w][0], b1 = ranges[w][1];
     const int64_t len = b1 - b0;
     const bool staged = len <= kMergeLds;
     if (staged) {
