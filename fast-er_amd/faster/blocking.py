@@ -109,6 +109,9 @@ class BlockedLinkage:
         self.n_patterns = 3 ** self.n_fuzzy * 2 ** self.n_exact
         self._exact_totals = exact_totals_per_block([a for a, _ in self._ex_host], [b for _, b in self._ex_host],
                                                     self.block_a, self.block_b, self.n_blocks)
+        # Record pairs per block (rows of A x rows of B), fixed for the object.
+        self._block_pairs = (np.bincount(self.block_a, minlength=self.n_blocks).astype(np.int64)
+                             * np.bincount(self.block_b, minlength=self.n_blocks).astype(np.int64))
         self._plan = None
         self.tuning = dict(DEFAULT_TUNING)
         self.last_ms = None
@@ -186,9 +189,7 @@ class BlockedLinkage:
             listed[:, c % low] += hist[:, c]
         for d in range(1, low):
             counts[:, d] = self._exact_totals[d] - listed[:, d]
-        na = np.bincount(self.block_a, minlength=self.n_blocks).astype(np.int64)
-        nb = np.bincount(self.block_b, minlength=self.n_blocks).astype(np.int64)
-        counts[:, 0] = na * nb - counts[:, 1:].sum(axis=1)
+        counts[:, 0] = self._block_pairs - counts[:, 1:].sum(axis=1)
         self.Block_Counts = counts
         return counts
 
