@@ -414,7 +414,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   args.hmask = 0;
   args.code_stride = 0;
   args.overflow = nullptr;
-  args.n_candidates = nullptr;
+  args.n_candidates = bound ? nullptr : counter + 1;  // survivors the bit-parallel pass scores
   args.cls_start = d_cls;
   args.cls_tile0 = d_cls ? d_cls + kNumClasses : nullptr;
   // The bit-parallel form scores its (rare) survivors inside its VALU-bound
@@ -565,7 +565,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   uint64_t h_count[2] = {0, 0};
   FH_HIP(hipMemcpyAsync(h_count, counter, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   FH_HIP(hipStreamSynchronize(s));
-  if (stats) stats->pairs_candidates = n_candidates;
+  if (stats) stats->pairs_candidates = bound ? n_candidates : (int64_t)h_count[1];
   if (stats) {
     float ms = 0;
     FH_HIP(hipEventElapsedTime(&ms, ev0, ev1));

@@ -468,6 +468,9 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
     a.planes_a = nullptr;
     a.planes_b = nullptr;
     a.n_candidates = nullptr;
+    a.cand = nullptr;
+    a.cls_start = nullptr;  // tiles are per block: survivors are scored byte-wise
+    a.cls_tile0 = nullptr;
     a.hcodes = pl->hcodes;
     a.hmask = slots - 1;
     a.overflow = pl->overflow;

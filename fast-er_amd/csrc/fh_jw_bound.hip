@@ -14,7 +14,7 @@
 // v_mfma_i32_32x32x32_i8 pair per 32 x 32 block of pairs. A pair can reach
 // level 1 only if m >= mmin(len1, len2) (the host table of fh_jw_core.h), so
 // pairs with bound < mmin are level 0; the rest go to the exact restatement
-// of the reference function (eval_pair). Results equal the bit-parallel
+// of the reference function (candidate_level). Results equal the bit-parallel
 // form's exactly: both apply the same exact scorer to supersets of the pairs
 // that reach level 1.
 //
@@ -115,91 +115,19 @@ __device__ __forceinline__ void push_candidate(const JwArgs& a, uint64_t* q, int
   qn = __builtin_amdgcn_readfirstlane(qn + (int)__popcll(mask));
 }
 
-// Exact scoring of the collected candidates, one per thread, from string 2's
-// position masks in the bit-parallel table (the complemented word of row
-// code(c), at the string's column, has bit j clear iff byte j is c):
-//   * greedy match (faster/comparison.py:48-64): for character i of string 1
-//     the lowest in-window, unused position of its row is taken;
-//   * pairs with m < mmin stop here (level 0);
-//   * transpositions (:73-99): the k-th matched character of string 1 is
-//     compared with the k-th matched position of string 2 by testing that
-//     position in the character's row;
-//   * prefix (:106-123): position i of string 2 holds string-1 byte i iff bit
-//     i of that byte's row is set; then the reference's formula.
-// Bytes of string 2 are never read. In deduplication the lexicographically
-// larger value is string 1 (faster/deduplication.py); when that is the
-// string-2 side the pair goes to the byte-wise restatement (eval_pair).
-__device__ __forceinline__ uint64_t row_mask(const uint32_t* base, int tw, int code, int kind, int col, int shift,
-                                             uint64_t lmask) {
-  const uint32_t* row = base + (size_t)code * tw + col;
-  const uint64_t w = kind == kW64 ? ~(((uint64_t)row[1] << 32) | row[0]) : (uint64_t)(~row[0] >> shift);
-  return w & lmask;
-}
-
 __global__ void __launch_bounds__(256) k_eval_candidates(const JwArgs* __restrict__ ap, int64_t n) {
   const JwArgs& a = *ap;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t v = a.cand[e];
-    const int64_t sa = (int64_t)(v >> 32), sb = (int64_t)(v & 0xFFFFFFFFull);
-    const uint64_t ea = a.sorted_a[sa], eb = a.sorted_b[sb];
-    const int64_t oa = a.off_a[ea];
-    const int la = (int)(a.off_a[ea + 1] - oa);
-    const int64_t ob = a.off_b[eb];
-    const int lb = (int)(a.off_b[eb + 1] - ob);
-    const int mm = a.mmin[la * (kLongLen + 1) + lb];
-    const int kind = kind_of_len(lb);
-    const int64_t pos = sb - a.cls_start[lb];
-    const int per = kind_tile_strings(kind, a.tw);
-    const int64_t tile = a.cls_tile0[lb] + pos / per;
-    const int p = (int)(pos % per);
-    const uint32_t* base = a.table + (size_t)tile * a.krows * a.tw;
-    int col = p, shift = 0;
-    if (packed_kind(kind)) {
-      col = p / kind_lanes(kind);
-      shift = kind_lane_shift(kind, p % kind_lanes(kind));
-    } else if (kind == kW64) {
-      col = 2 * p;
+  // Wave-uniform trip count, so every lane reaches the per-wave emission.
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e0 = (int64_t)blockIdx.x * blockDim.x; e0 < n; e0 += stride) {
+    const int64_t e = e0 + threadIdx.x;
+    int lvl = 0;
+    uint64_t x = 0, y = 0;
+    if (e < n) {
+      const uint64_t v = a.cand[e];
+      lvl = candidate_level(a, (int64_t)(v >> 32), (int64_t)(v & 0xFFFFFFFFull), x, y);
     }
-    const uint8_t* s1 = a.chars_a + oa;
-    const int w = max(la, lb) / 2 - 1;
-    const uint64_t lmask = lb >= 64 ? ~0ull : ((1ull << lb) - 1ull);
-    uint64_t used = 0, flag1 = 0;
-#pragma unroll 4
-    for (int i = 0; i < la; ++i) {
-      uint64_t avail = row_mask(base, a.tw, a.amap[s1[i]], kind, col, shift, lmask);
-      avail &= window_bits(i, w, lb) & ~used;
-      if (avail) {
-        used |= avail & (~avail + 1ull);
-        flag1 |= 1ull << i;
-      }
-    }
-    const int m = __popcll(used);
-    if (m < mm) continue;
-    if (a.mode == FH_MODE_DEDUP && !lex_greater(s1, la, a.chars_b + ob, lb)) {
-      eval_pair(a, ea, eb);
-      continue;
-    }
-    float s = 0.0f;
-    if (m > 0) {
-      int tt = 0;
-      uint64_t f = flag1, u = used;
-      for (int k = 0; k < m; ++k) {
-        const int i = __builtin_ctzll(f), j = __builtin_ctzll(u);
-        if (!((row_mask(base, a.tw, a.amap[s1[i]], kind, col, shift, lmask) >> j) & 1ull)) ++tt;
-        f &= f - 1;
-        u &= u - 1;
-      }
-      int pre = 0;
-      const int lim = min(min(la, lb), 4);
-      for (int i = 0; i < lim; ++i) {
-        if ((row_mask(base, a.tw, a.amap[s1[i]], kind, col, shift, lmask) >> i) & 1ull)
-          ++pre;
-        else
-          break;
-      }
-      s = jw_formula(la, lb, m, tt, pre, a.p);
-    }
-    emit_level(a, ea, eb, level_of(s, a.lo, a.hi));
+    emit_levels_wave(a, x, y, lvl);
   }
 }
 

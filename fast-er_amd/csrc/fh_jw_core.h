@@ -26,14 +26,14 @@
 //         nc = nmask | nwin_i | navail        (v_bitop3: candidates = zeros)
 //         t  = nc + 1                          (lowest candidate -> 1)
 //         navail = (nc & navail) | (~nc & t)   (v_bitop3: mark it used)
-//     Short string-2 masks are packed several to a dword, each in a lane of
-//     LB bits whose bit 0 is a guard that is always set in nc: lanes of 8
-//     bits (strings of <= 7 bytes, 4 per dword), 10 bits (<= 9 bytes, 3 per
-//     dword) or 16 bits (<= 15 bytes, 2 per dword). One 32-bit add of 1 per
-//     lane then advances every lane at once: a lane whose bits are all set
-//     carries into the next lane's guard, which absorbs it (the guard is set
-//     in nc, so the select never copies it), and no carry reaches a
-//     position bit of another lane.
+//     Short string-2 masks are packed several to a dword, each lane's
+//     positions preceded by a guard bit that is always set in nc: lanes of 8
+//     bits (strings of <= 7 bytes, 4 per dword), 11 bits (<= 10 bytes, 3 per
+//     dword, no guard under the lowest lane) or 16 bits (<= 15 bytes, 2 per
+//     dword). One 32-bit add of 1 per lane then advances every lane at once:
+//     a lane whose bits are all set carries into the next lane's guard, which
+//     absorbs it (the guard is set in nc, so the select never copies it), and
+//     no carry reaches a position bit of another lane.
 //   * The match count m = popcount of used positions is an exact count for the
 //     greedy match. A host-built table gives, for every (len1, len2), the
 //     smallest m for which any transposition count and prefix can reach level
@@ -353,7 +353,6 @@ __device__ __forceinline__ void hash_add(const JwArgs& a, uint64_t id, uint32_t 
   atomicOr(a.overflow, 1u);
 }
 
-// Evaluates one queued pair exactly and emits its key when its level is > 0.
 // Emits the level key of pair (x = string 1, y = string 2) when its level is > 0.
 __device__ __forceinline__ void emit_level(const JwArgs& a, uint64_t x, uint64_t y, int lvl) {
   if (lvl <= 0) return;
@@ -366,31 +365,145 @@ __device__ __forceinline__ void emit_level(const JwArgs& a, uint64_t x, uint64_t
   }
 }
 
-__device__ __noinline__ void eval_pair(const JwArgs& a, uint64_t ea, uint64_t eb) {
+// The same for every active lane of a wave at once (all of them must call
+// it): the appended keys take one atomic per wave instead of one per key.
+__device__ __forceinline__ void emit_levels_wave(const JwArgs& a, uint64_t x, uint64_t y, int lvl) {
+  if (a.hkeys) {
+    emit_level(a, x, y, lvl);
+    return;
+  }
+  const uint64_t mask = __ballot(lvl > 0);
+  if (mask == 0) return;
+  unsigned long long base = 0;
+  const int first = __builtin_ctzll(mask);
+  if ((int)__lane_id() == first) base = atomicAdd(a.counter, (unsigned long long)__popcll(mask));
+  base = __shfl(base, first, 64);
+  if (lvl > 0) {
+    const unsigned long long pos = base + (unsigned long long)__popcll(mask & fhd::lanemask_lt());
+    if ((int64_t)pos < a.cap) a.hits[pos] = ((uint64_t)x * (uint64_t)a.n_b + y) * 4ull + (uint64_t)lvl;
+  }
+}
+
+// Window of string-2 positions [i - w, i + w] clipped to [0, lenB).
+__device__ __forceinline__ uint64_t window_bits(int i, int w, int lenB) {
+  const int lo_i = max(0, i - w);
+  const int hi_i = min(lenB - 1, i + w);
+  if (hi_i < lo_i) return 0ull;
+  const int width = hi_i - lo_i + 1;
+  return (width >= 64 ? ~0ull : ((1ull << width) - 1ull)) << lo_i;
+}
+
+// Level of one pair by the byte-wise restatement (jw_exact), plus 4 when
+// string 1 is the B-side value eb (in deduplication the lexicographically
+// larger value is string 1, faster/deduplication.py).
+constexpr int kSwapped = 4;
+__device__ __noinline__ int pair_level(const JwArgs& a, uint64_t ea, uint64_t eb) {
   const int64_t oa = a.off_a[ea];
   const int la = (int)(a.off_a[ea + 1] - oa);
   const int64_t ob = a.off_b[eb];
   const int lb = (int)(a.off_b[eb + 1] - ob);
   const uint8_t* sa = a.chars_a + oa;
   const uint8_t* sb = a.chars_b + ob;
-  uint64_t x = ea, y = eb;
-  float s;
-  if (a.mode == FH_MODE_DEDUP && !lex_greater(sa, la, sb, lb)) {
-    s = jw_exact<1>(sb, lb, sa, la, a.p);
+  if (a.mode == FH_MODE_DEDUP && !lex_greater(sa, la, sb, lb))
+    return level_of(jw_exact<1>(sb, lb, sa, la, a.p), a.lo, a.hi) + kSwapped;
+  return level_of(jw_exact<1>(sa, la, sb, lb, a.p), a.lo, a.hi);
+}
+
+// Exact score of one pair (sorted indices sa, sb), from string 2's
+// position masks in the bit-parallel table (the complemented word of row
+// code(c), at the string's column, has bit j clear iff byte j is c):
+//   * greedy match (faster/comparison.py:48-64): for character i of string 1
+//     the lowest in-window, unused position of its row is taken;
+//   * pairs with m < mmin stop here (level 0);
+//   * transpositions (:73-99): the k-th matched character of string 1 is
+//     compared with the k-th matched position of string 2 by testing that
+//     position in the character's row;
+//   * prefix (:106-123): position i of string 2 holds string-1 byte i iff bit
+//     i of that byte's row is set; then the reference's formula.
+// Bytes of string 2 are never read. In deduplication the lexicographically
+// larger value is string 1 (faster/deduplication.py); when that is the
+// string-2 side the pair goes to the byte-wise restatement (pair_level).
+__device__ __forceinline__ uint64_t row_mask(const uint32_t* base, int tw, int code, int kind, int col, int shift,
+                                             uint64_t lmask) {
+  const uint32_t* row = base + (size_t)code * tw + col;
+  const uint64_t w = kind == kW64 ? ~(((uint64_t)row[1] << 32) | row[0]) : (uint64_t)(~row[0] >> shift);
+  return w & lmask;
+}
+
+__device__ __forceinline__ int candidate_level(const JwArgs& a, int64_t sa, int64_t sb, uint64_t& x, uint64_t& y) {
+  const uint64_t ea = a.sorted_a[sa], eb = a.sorted_b[sb];
+  x = ea;
+  y = eb;
+  const int64_t oa = a.off_a[ea];
+  const int la = (int)(a.off_a[ea + 1] - oa);
+  const int64_t ob = a.off_b[eb];
+  const int lb = (int)(a.off_b[eb + 1] - ob);
+  const int mm = a.mmin[la * (kLongLen + 1) + lb];
+  const int kind = kind_of_len(lb);
+  const int64_t pos = sb - a.cls_start[lb];
+  const int per = kind_tile_strings(kind, a.tw);
+  const int64_t tile = a.cls_tile0[lb] + pos / per;
+  const int p = (int)(pos % per);
+  const uint32_t* base = a.table + (size_t)tile * a.krows * a.tw;
+  int col = p, shift = 0;
+  if (packed_kind(kind)) {
+    col = p / kind_lanes(kind);
+    shift = kind_lane_shift(kind, p % kind_lanes(kind));
+  } else if (kind == kW64) {
+    col = 2 * p;
+  }
+  const uint8_t* s1 = a.chars_a + oa;
+  const int w = max(la, lb) / 2 - 1;
+  const uint64_t lmask = lb >= 64 ? ~0ull : ((1ull << lb) - 1ull);
+  uint64_t used = 0, flag1 = 0;
+#pragma unroll 4
+  for (int i = 0; i < la; ++i) {
+    uint64_t avail = row_mask(base, a.tw, a.amap[s1[i]], kind, col, shift, lmask);
+    avail &= window_bits(i, w, lb) & ~used;
+    if (avail) {
+      used |= avail & (~avail + 1ull);
+      flag1 |= 1ull << i;
+    }
+  }
+  const int m = __popcll(used);
+  if (m < mm) return 0;
+  if (a.mode == FH_MODE_DEDUP && !lex_greater(s1, la, a.chars_b + ob, lb)) {
+    const int code = pair_level(a, ea, eb);
     x = eb;
     y = ea;
-  } else {
-    s = jw_exact<1>(sa, la, sb, lb, a.p);
+    return code - kSwapped;
   }
-  emit_level(a, x, y, level_of(s, a.lo, a.hi));
+  float s = 0.0f;
+  if (m > 0) {
+    int tt = 0;
+    uint64_t f = flag1, u = used;
+    for (int k = 0; k < m; ++k) {
+      const int i = __builtin_ctzll(f), j = __builtin_ctzll(u);
+      if (!((row_mask(base, a.tw, a.amap[s1[i]], kind, col, shift, lmask) >> j) & 1ull)) ++tt;
+      f &= f - 1;
+      u &= u - 1;
+    }
+    int pre = 0;
+    const int lim = min(min(la, lb), 4);
+    for (int i = 0; i < lim; ++i) {
+      if ((row_mask(base, a.tw, a.amap[s1[i]], kind, col, shift, lmask) >> i) & 1ull)
+        ++pre;
+      else
+        break;
+    }
+    s = jw_formula(la, lb, m, tt, pre, a.p);
+  }
+  return level_of(s, a.lo, a.hi);
 }
 
 // Processes the wave's queue of (string-1 lane, string-2 sorted index) pairs.
 // With a candidate buffer (a.cand), the queue is appended to it as
 // (string-1 sorted index << 32) | string-2 sorted index, with one atomic per
 // flush, for a separate scoring kernel; entries past its capacity are
-// counted only. Without one (the persistent blocked kernel), the pairs are
-// scored here.
+// counted only. Without one, the pairs are scored here, byte-wise (for
+// survivors, which all need the full score, this measured faster than the
+// table-based scorer of the candidate kernel), and their keys appended with
+// one atomic per wave.
 __device__ __noinline__ void flush_queue(const JwArgs& a, const uint64_t* q, int qn, uint64_t s0a) {
   if (a.cand) {
     unsigned long long base = 0;
@@ -403,11 +516,21 @@ __device__ __noinline__ void flush_queue(const JwArgs& a, const uint64_t* q, int
     }
     return;
   }
-  for (int k = (int)__lane_id(); k < qn; k += 64) {
-    const uint64_t v = q[k];
-    const uint64_t ea = a.sorted_a[s0a + (v & 63ull)];
-    const uint64_t eb = a.sorted_b[v >> 6];
-    eval_pair(a, ea, eb);
+  if (a.n_candidates && __lane_id() == 0) atomicAdd(a.n_candidates, (unsigned long long)qn);
+  for (int k0 = 0; k0 < qn; k0 += 64) {
+    const int k = k0 + (int)__lane_id();
+    int lvl = 0;
+    uint64_t x = 0, y = 0;
+    if (k < qn) {
+      const uint64_t v = q[k];
+      const uint64_t ea = a.sorted_a[s0a + (v & 63ull)], eb = a.sorted_b[v >> 6];
+      const int code = pair_level(a, ea, eb);
+      const bool swapped = code >= kSwapped;
+      lvl = code & 3;
+      x = swapped ? eb : ea;
+      y = swapped ? ea : eb;
+    }
+    emit_levels_wave(a, x, y, lvl);
   }
 }
 
@@ -484,14 +607,6 @@ __device__ __forceinline__ uint32_t row_off_rt(const uint32_t (&o)[N], int i) {
   return (i & 1) ? (w >> 16) : (w & 0xFFFFu);
 }
 
-// Window of string-2 positions [i - w, i + w] clipped to [0, lenB).
-__device__ __forceinline__ uint64_t window_bits(int i, int w, int lenB) {
-  const int lo_i = max(0, i - w);
-  const int hi_i = min(lenB - 1, i + w);
-  if (hi_i < lo_i) return 0ull;
-  const int width = hi_i - lo_i + 1;
-  return (width >= 64 ? ~0ull : ((1ull << width) - 1ull)) << lo_i;
-}
 
 // Complemented window of string-1 character i in every lane of a packed
 // kind: the lanes' positions inside the window are clear; the guards, the
