@@ -3,6 +3,7 @@
 // fh_jw_core.h.
 #include <memory>
 
+#define FH_JW_PAIR_TABLE 1
 #include "fh_jw_core.h"
 
 // ===========================================================================

@@ -504,6 +504,24 @@ __device__ __forceinline__ int candidate_level(const JwArgs& a, int64_t sa, int6
 // survivors, which all need the full score, this measured faster than the
 // table-based scorer of the candidate kernel), and their keys appended with
 // one atomic per wave.
+// The persistent blocked kernel's form: hits accumulate in the pair table
+// (hash_add, one slot per pair), so each lane scores and adds its own pairs;
+// kept apart so the blocked kernel does not carry the list form's registers.
+__device__ __noinline__ void score_and_add(const JwArgs& a, uint64_t ea, uint64_t eb) {
+  const int code = pair_level(a, ea, eb);
+  if (code >= kSwapped)
+    emit_level(a, eb, ea, code & 3);
+  else
+    emit_level(a, ea, eb, code);
+}
+
+__device__ __noinline__ void flush_queue_hash(const JwArgs& a, const uint64_t* q, int qn, uint64_t s0a) {
+  for (int k = (int)__lane_id(); k < qn; k += 64) {
+    const uint64_t v = q[k];
+    score_and_add(a, a.sorted_a[s0a + (v & 63ull)], a.sorted_b[v >> 6]);
+  }
+}
+
 __device__ __noinline__ void flush_queue(const JwArgs& a, const uint64_t* q, int qn, uint64_t s0a) {
   if (a.cand) {
     unsigned long long base = 0;
@@ -534,13 +552,28 @@ __device__ __noinline__ void flush_queue(const JwArgs& a, const uint64_t* q, int
   }
 }
 
+// The persistent blocked kernel, compiled in a translation unit of its own
+// with FH_JW_PAIR_TABLE set, accumulates hits in the pair table and uses
+// only flush_queue_hash; the plain passes use only flush_queue. Each kernel
+// then carries the registers of its own form alone.
+#ifndef FH_JW_PAIR_TABLE
+#define FH_JW_PAIR_TABLE 0
+#endif
+__device__ __forceinline__ void flush_pairs(const JwArgs& a, const uint64_t* q, int qn, uint64_t s0a) {
+#if FH_JW_PAIR_TABLE
+  flush_queue_hash(a, q, qn, s0a);
+#else
+  flush_queue(a, q, qn, s0a);
+#endif
+}
+
 // Pushes one survivor per lane where `pred` holds; flushes first when full.
 __device__ __forceinline__ void queue_push(const JwArgs& a, uint64_t* q, int& qn, bool pred, uint64_t sb,
                                            uint64_t s0a) {
   const uint64_t mask = __ballot(pred);
   if (mask == 0) return;
   if (qn + 64 > kQCap) {
-    flush_queue(a, q, qn, s0a);
+    flush_pairs(a, q, qn, s0a);
     qn = 0;
   }
   if (pred) {
@@ -930,7 +963,7 @@ __device__ __forceinline__ void run_item(const JwArgs& a, uint32_t* lds, int64_t
       tile_pass_rolled<KIND, LMAX>(a, c, off, lenA);
     }
   }
-  if (wave_ok && c.qn > 0) flush_queue(a, c.q, c.qn, (uint64_t)wm.s0);
+  if (wave_ok && c.qn > 0) flush_pairs(a, c.q, c.qn, (uint64_t)wm.s0);
 }
 
 // LMIN..LMAX: the string-1 lengths this instance handles.
