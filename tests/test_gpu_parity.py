@@ -42,6 +42,42 @@ def test_library_and_device():
     assert torch.cuda.is_available()
 
 
+def test_valu_probe():
+    """fh_probe_valu (the VALU-peak check of bench.py --valu-probe) reports
+    the lane-ops it launched, and its per-thread words equal the recurrence
+    restated on the CPU (or3, add, select on 8 chains per thread)."""
+    import ctypes
+
+    import faster._lib as L
+
+    lib = L.load()
+    blocks, iters = 3, 5
+    out = torch.zeros(blocks * 256, dtype=torch.int32, device="cuda")
+    ops = ctypes.c_double(0.0)
+    L.check(lib.fh_probe_valu(L.ptr(out), blocks, iters, ctypes.byref(ops), L.stream()), "fh_probe_valu")
+    torch.cuda.synchronize()
+    assert ops.value == 3 * 8 * 8 * iters * blocks * 256
+    seed = 0x9E3779B9
+    tid = np.arange(256, dtype=np.uint32)
+    m = [np.uint32((seed * (k + 1)) & 0xFFFFFFFF) ^ tid for k in range(8)]
+    na = [np.zeros(256, dtype=np.uint32) for _ in range(8)]
+    w = np.uint32(seed >> 7)
+    for _ in range(iters):
+        for s in range(8):
+            for u in range(8):
+                nc = m[(s + u) % 8] | w | na[u]
+                t = nc + np.uint32(0x01010101)
+                na[u] = (nc & na[u]) | (~nc & t)
+    want = np.zeros(256, dtype=np.uint32)
+    for x in na:
+        want ^= x
+    got = out.cpu().numpy().view(np.uint32).reshape(blocks, 256)
+    for b in range(blocks):
+        np.testing.assert_array_equal(got[b], want)
+    with pytest.raises(RuntimeError, match="bad arguments"):
+        L.check(lib.fh_probe_valu(L.ptr(out), 0, iters, ctypes.byref(ops), L.stream()), "fh_probe_valu")
+
+
 def test_kat_levels(oracle):
     from faster.comparison import jaro_winkler_gpu
 
