@@ -133,8 +133,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (it would shard by A-row "
-              "ranges, see DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: there is no multi-GPU path (it would shard by contiguous "
+              "A-row ranges); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
