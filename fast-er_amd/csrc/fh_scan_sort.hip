@@ -5,7 +5,8 @@
 // wavefronts: the block scan combines wave-level shuffles with one LDS
 // exchange, and the radix scatter ranks keys inside a wave with eight ballots
 // (a match-any over the digit bits) before combining waves through LDS, which
-// keeps the sort stable without per-thread digit counters.
+// keeps the sort stable without per-thread digit counters; it then stages the
+// digit-sorted tile in LDS so that each digit's keys are written as one run.
 #include "fh_common.h"
 
 namespace fh {
@@ -146,6 +147,20 @@ __global__ void __launch_bounds__(kSortThreads) k_rs_hist(const K* __restrict__ 
   hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
+// One tile of kSortTile keys per workgroup. Wave w owns the tile's elements
+// [w * kWaveItems, (w + 1) * kWaveItems) and reads them 64 at a time, so the
+// tile's element order is (wave, round, lane) and every read is one
+// contiguous 512-byte (u64) or 256-byte (u32) run. Each wave ranks its keys
+// within their digit with eight ballots and one LDS atomic per digit group on
+// a wave-private digit counter; the waves' counts then combine into each
+// key's slot in the digit-sorted tile, which is staged in LDS and written
+// out in slot order: keys of one digit leave as one contiguous run instead of
+// one scattered write per key. Values are not staged: each slot keeps the
+// key's position in the tile (16 bits), and the value is gathered from the
+// tile's 32 KB of values, which the L2 holds, as it is written out.
+constexpr int kWaveItems = kSortTile / kSortWaves;  // 1024
+constexpr int kWaveRounds = kWaveItems / 64;       // 16
+
 template <typename K, bool kVals>
 __global__ void __launch_bounds__(kSortThreads) k_rs_scatter(const K* __restrict__ keys_in,
                                                             const uint64_t* __restrict__ vals_in, int64_t n,
@@ -153,25 +168,32 @@ __global__ void __launch_bounds__(kSortThreads) k_rs_scatter(const K* __restrict
                                                             const uint32_t* __restrict__ hist_scanned,
                                                             K* __restrict__ keys_out,
                                                             uint64_t* __restrict__ vals_out) {
-  __shared__ uint32_t goff[256];
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wcnt[kSortWaves][256];
+  static_assert(kSortThreads == kScanThreads, "the digit scan uses the block scan");
+  __shared__ uint32_t goff[256];              // first output index of digit d for this tile
+  __shared__ uint32_t loff[256];              // first tile slot of digit d
+  __shared__ uint32_t wcnt[kSortWaves][256];  // per-wave digit counts, then per-wave starts
+  __shared__ K skeys[kSortTile];
+  __shared__ uint16_t sidx[kVals ? kSortTile : 1];  // tile position of the key in each slot
   const int tid = threadIdx.x;
-  const int wave = tid / 64;
+  const int wave = tid >> 6;
+  const int lane = tid & 63;
   goff[tid] = hist_scanned[(int64_t)tid * ntiles + blockIdx.x];
-  run[tid] = 0;
 #pragma unroll
   for (int w = 0; w < kSortWaves; ++w) wcnt[w][tid] = 0;
   __syncthreads();
+
   const uint64_t lt = fhd::lanemask_lt();
-  const int64_t base = (int64_t)blockIdx.x * kSortTile;
-  for (int r = 0; r < kSortItems; ++r) {
-    const int64_t e = base + (int64_t)r * kSortThreads + tid;
+  const int64_t tile0 = (int64_t)blockIdx.x * kSortTile;
+  const int64_t wbase = tile0 + (int64_t)wave * kWaveItems;
+  uint32_t* cnt = wcnt[wave];
+  K key[kWaveRounds];
+  uint32_t rnk[kWaveRounds];
+#pragma unroll
+  for (int r = 0; r < kWaveRounds; ++r) {
+    const int64_t e = wbase + r * 64 + lane;
     const bool valid = e < n;
-    K key = valid ? keys_in[e] : K(0);
-    uint64_t val = 0;
-    if (kVals && valid) val = vals_in[e];
-    const uint32_t d = (uint32_t)((key >> shift) & 255u);
+    key[r] = valid ? keys_in[e] : K(0);
+    const uint32_t d = (uint32_t)((key[r] >> shift) & 255u);
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -180,26 +202,46 @@ __global__ void __launch_bounds__(kSortThreads) k_rs_scatter(const K* __restrict
       peers &= bit ? bal : ~bal;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & lt);
-    if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    if (valid) {
-      uint32_t pre = run[d];
-      for (int w = 0; w < wave; ++w) pre += wcnt[w][d];
-      const uint32_t pos = goff[d] + pre + rank;
-      keys_out[pos] = key;
-      if (kVals) vals_out[pos] = val;
-    }
-    __syncthreads();
-    {
-      uint32_t add = 0;
+    // The first lane of each digit group adds the group's size to the wave's
+    // counter and hands the old value to its group. The wave's LDS atomics
+    // complete in issue order, so the rounds need no wait between them.
+    uint32_t old = 0;
+    if (valid && rank == 0) old = atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
+    const int leader = valid ? __ffsll((unsigned long long)peers) - 1 : lane;
+    rnk[r] = (uint32_t)__shfl(old, leader, 64) + rank;
+  }
+  __syncthreads();
+  {
+    uint32_t run = 0;
 #pragma unroll
-      for (int w = 0; w < kSortWaves; ++w) {
-        add += wcnt[w][tid];
-        wcnt[w][tid] = 0;
-      }
-      run[tid] += add;
+    for (int w = 0; w < kSortWaves; ++w) {
+      const uint32_t t = wcnt[w][tid];
+      wcnt[w][tid] = run;
+      run += t;
     }
-    __syncthreads();
+    uint32_t all;
+    loff[tid] = block_exclusive_scan<uint32_t>(run, &all);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kWaveRounds; ++r) {
+    const int64_t e = wbase + r * 64 + lane;
+    if (e < n) {
+      const uint32_t d = (uint32_t)((key[r] >> shift) & 255u);
+      const uint32_t slot = loff[d] + wcnt[wave][d] + rnk[r];
+      skeys[slot] = key[r];
+      if (kVals) sidx[slot] = (uint16_t)(wave * kWaveItems + r * 64 + lane);
+    }
+  }
+  __syncthreads();
+  const int tile_n = (int)min((int64_t)kSortTile, n - tile0);
+#pragma unroll 4
+  for (int j = tid; j < tile_n; j += kSortThreads) {
+    const K k = skeys[j];
+    const uint32_t d = (uint32_t)((k >> shift) & 255u);
+    const uint32_t pos = goff[d] + ((uint32_t)j - loff[d]);
+    keys_out[pos] = k;
+    if (kVals) vals_out[pos] = vals_in[tile0 + sidx[j]];
   }
 }
 
