@@ -288,12 +288,17 @@ __global__ void __launch_bounds__(kThreads) k_member_mask(const int64_t* __restr
 // pair when no earlier variable lists the pair; owners get their pattern code,
 // the others 0xFFFFFFFF.
 //
-// Each block handles a tile of consecutive source entries. For every other
-// list it locates the sub-range of keys that can match the tile's id range
-// with two binary searches; when the sub-range fits in LDS it is staged there
-// and searched locally, otherwise entries search the sub-range in place.
+// Each block handles a tile of consecutive source entries, whose ids it keeps
+// in LDS, and one range of every other list: the lists are split at the ids
+// of the tiles' first entries by a separate search kernel. The shorter side
+// of each match-up searches the longer one: entries of a range no longer
+// than the tile binary-search the tile's ids; a longer range is staged in LDS
+// a chunk at a time and the tile's entries search it, or, when it is many
+// times longer than the tile, searched in place. Codes accumulate in LDS with
+// atomic adds, as several lists' matches land on one entry.
 constexpr int kMergeTile = 1024;
-constexpr int kMergeLds = 4096;
+constexpr int kMergeLds = 2048;
+constexpr int kMergeInPlace = 4 * kMergeLds;  // longer ranges are searched in place
 constexpr int kMaxVars = 16;
 
 struct MergeArgs {
@@ -321,113 +326,127 @@ __device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* a, int64_t lo
   return lo;
 }
 
-// lower_bound over sorted a[lo, hi) by the 32 lanes of one half-wave (`half`
-// = 0 or 1, all 32 lanes call it with the same arguments): each round probes
-// 32 evenly spaced positions with one load each and keeps the interval
-// between the last probe below x and the first at or above it, so a search
-// of n entries takes about log32(n) dependent loads instead of log2(n).
-__device__ int64_t lower_bound_u64_half(const uint64_t* a, int64_t lo, int64_t hi, uint64_t x, int half) {
-  const int j = (int)(__lane_id() & 31);
-  const uint64_t half_mask = half ? 0xFFFFFFFF00000000ull : 0x00000000FFFFFFFFull;
-  while (hi - lo > 32) {
-    const int64_t step = (hi - lo + 31) / 32;
-    const int64_t p = lo + step * j;
-    const bool below = p < hi && a[p] < x;
-    const int k = __popcll(__ballot(below) & half_mask);  // probes below x: positions 0 .. k-1
-    const int64_t nlo = k > 0 ? lo + step * (k - 1) + 1 : lo;
-    const int64_t nhi = min(hi, lo + step * k);
-    lo = nlo;
-    hi = nhi;
+// bounds[w * (ntiles + 1) + t]: the first entry of list w whose pair id is at
+// least the id of the first entry of tile t of list v, and list w's size for
+// t = ntiles. Consecutive bounds split every other list into one range per
+// tile of v. One thread per search: the dependent loads of many searches
+// overlap, where a search inside the tile kernel would stall its workgroup.
+__global__ void __launch_bounds__(kThreads) k_tile_bounds(MergeArgs m, int v, int64_t ntiles,
+                                                          int64_t* __restrict__ bounds) {
+  const int64_t per = ntiles + 1;
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= per * m.n_vars) return;
+  const int w = (int)(g / per);
+  const int64_t t = g - (int64_t)w * per;
+  int64_t b = 0;
+  if (w != v) {
+    b = t == ntiles ? m.sizes[w]
+                    : lower_bound_u64(m.lists[w], 0, m.sizes[w], (m.lists[v][t * kMergeTile] >> 2) * 4ull);
   }
-  const int64_t p = lo + j;
-  const bool below = p < hi && a[p] < x;
-  return lo + __popcll(__ballot(below) & half_mask);
+  bounds[g] = b;
 }
 
-__global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, uint32_t* __restrict__ codes) {
+__global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, const int64_t* __restrict__ bounds,
+                                                            uint32_t* __restrict__ codes) {
   __shared__ uint64_t seg[kMergeLds];
-  __shared__ int64_t ranges[kMaxVars][2];
+  __shared__ uint64_t tids[kMergeTile];   // the tile's pair ids, ascending
+  __shared__ uint32_t tcode[kMergeTile];  // pattern code so far of each tile entry
+  __shared__ uint8_t tgone[kMergeTile];   // 1: an earlier variable lists the pair
   const uint64_t* src = m.lists[v];
   const int64_t n = m.sizes[v];
   const int64_t t0 = (int64_t)blockIdx.x * kMergeTile;
   const int64_t t1 = min(t0 + (int64_t)kMergeTile, n);
-  constexpr int kPer = kMergeTile / kThreads;
-  uint64_t id[kPer];
-  uint32_t code[kPer];
-  bool owner[kPer];
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int64_t e = t0 + k * kThreads + threadIdx.x;
-    const uint64_t key = e < t1 ? src[e] : 0ull;
-    id[k] = key >> 2;
-    code[k] = (uint32_t)(key & 3ull) * m.stride[v];
-    owner[k] = e < t1;
-  }
-  const uint64_t id_lo = src[t0] >> 2;
-  const uint64_t id_hi = src[t1 - 1] >> 2;
-  // The ranges of this tile's ids in every other list, all at once: wave q
-  // takes lists q, q + 4, ..., its lower half-wave the first key of the
-  // range and its upper half-wave the end.
-  {
-    const int wave = threadIdx.x >> 6;
-    const int half = (threadIdx.x >> 5) & 1;
-    for (int w = wave; w < m.n_vars; w += kThreads / 64) {
-      if (w == v) continue;
-      const uint64_t x = half ? id_hi * 4ull + 4ull : id_lo * 4ull;
-      const int64_t b = lower_bound_u64_half(m.lists[w], 0, m.sizes[w], x, half);
-      if ((threadIdx.x & 31) == 0) ranges[w][half] = b;
-    }
+  const int tn = (int)(t1 - t0);
+  const int64_t per = (int64_t)gridDim.x + 1;
+  for (int i = threadIdx.x; i < tn; i += kThreads) {
+    const uint64_t key = src[t0 + i];
+    tids[i] = key >> 2;
+    tcode[i] = (uint32_t)(key & 3ull) * m.stride[v];
+    tgone[i] = 0;
   }
   __syncthreads();
+  // Ranges no longer than the tile: each of their entries looks itself up
+  // among the tile's ids.
   for (int w = 0; w < m.n_vars; ++w) {
     if (w == v) continue;
-    const uint64_t* tl = m.lists[w];
-    const int64_t b0 = ranges[w][0], b1 = ranges[w][1];
-    const int64_t len = b1 - b0;
-    const bool staged = len <= kMergeLds;
-    if (staged) {
-      for (int64_t i = threadIdx.x; i < len; i += kThreads) seg[i] = tl[b0 + i];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      if (!owner[k] && code[k] == 0xFFFFFFFFu) continue;
-      const uint64_t x = id[k] * 4ull;
-      uint64_t hit;
-      bool found;
-      if (staged) {
-        const int64_t p = lower_bound_u64(seg, 0, len, x);
-        found = p < len && (seg[p] >> 2) == id[k];
-        hit = found ? seg[p] : 0ull;
-      } else {
-        const int64_t p = lower_bound_u64(tl, b0, b1, x);
-        found = p < b1 && (tl[p] >> 2) == id[k];
-        hit = found ? tl[p] : 0ull;
+    const int64_t b0 = bounds[w * per + blockIdx.x], len = bounds[w * per + blockIdx.x + 1] - b0;
+    if (len > tn) continue;
+    const uint64_t* tl = m.lists[w] + b0;
+    const uint32_t stride = m.stride[w];
+    for (int i = threadIdx.x; i < (int)len; i += kThreads) {
+      const uint64_t x = tl[i];
+      const uint64_t xid = x >> 2;
+      const int64_t p = lower_bound_u64(tids, 0, tn, xid);
+      if (p < tn && tids[p] == xid) {
+        if (w < v)
+          tgone[p] = 1;
+        else
+          atomicAdd(&tcode[p], (uint32_t)(x & 3ull) * stride);
       }
-      if (found) {
-        if (w < v) {
-          owner[k] = false;
-          code[k] = 0xFFFFFFFFu;
-        } else if (owner[k]) {
-          code[k] += (uint32_t)(hit & 3ull) * m.stride[w];
+    }
+  }
+  // Ranges far longer than the tile: each tile entry binary-searches the
+  // range in place, which touches about log2 of its length entries instead
+  // of reading all of it.
+  for (int w = 0; w < m.n_vars; ++w) {
+    if (w == v) continue;
+    const int64_t b0 = bounds[w * per + blockIdx.x], len = bounds[w * per + blockIdx.x + 1] - b0;
+    if (len <= kMergeInPlace) continue;
+    const uint64_t* tl = m.lists[w] + b0;
+    const uint32_t stride = m.stride[w];
+    for (int i = threadIdx.x; i < tn; i += kThreads) {
+      const uint64_t xid = tids[i];
+      const int64_t p = lower_bound_u64(tl, 0, len, xid * 4ull);
+      if (p < len) {
+        const uint64_t x = tl[p];
+        if ((x >> 2) == xid) {
+          if (w < v)
+            tgone[i] = 1;
+          else
+            atomicAdd(&tcode[i], (uint32_t)(x & 3ull) * stride);
         }
       }
     }
-    __syncthreads();
   }
-  if (m.n_imp > 0) {
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      if (!owner[k]) continue;
-      const uint64_t r1 = id[k] / m.id_div, r2 = id[k] % m.id_div;
-      for (int q = 0; q < m.n_imp; ++q)
-        if (m.imp_a[q][r1] == m.imp_b[q][r2]) code[k] += m.imp_stride[q];
+  // The ranges in between: staged through LDS a chunk at a time; each tile
+  // entry whose id lies within the chunk's ids looks itself up in the chunk.
+  for (int w = 0; w < m.n_vars; ++w) {
+    if (w == v) continue;
+    const int64_t b0 = bounds[w * per + blockIdx.x], len = bounds[w * per + blockIdx.x + 1] - b0;
+    if (len <= tn || len > kMergeInPlace) continue;
+    const uint64_t* tl = m.lists[w] + b0;
+    const uint32_t stride = m.stride[w];
+    for (int64_t c0 = 0; c0 < len; c0 += kMergeLds) {
+      const int clen = (int)min((int64_t)kMergeLds, len - c0);
+      __syncthreads();
+      for (int i = threadIdx.x; i < clen; i += kThreads) seg[i] = tl[c0 + i];
+      __syncthreads();
+      const uint64_t lo_id = seg[0] >> 2, hi_id = seg[clen - 1] >> 2;
+      for (int i = threadIdx.x; i < tn; i += kThreads) {
+        const uint64_t xid = tids[i];
+        if (xid < lo_id || xid > hi_id) continue;
+        const int64_t p = lower_bound_u64(seg, 0, clen, xid * 4ull);
+        if (p < clen && (seg[p] >> 2) == xid) {
+          if (w < v)
+            tgone[i] = 1;
+          else
+            atomicAdd(&tcode[i], (uint32_t)(seg[p] & 3ull) * stride);
+        }
+      }
     }
   }
-#pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int64_t e = t0 + k * kThreads + threadIdx.x;
-    if (e < t1) codes[e] = owner[k] ? code[k] : 0xFFFFFFFFu;
+  __syncthreads();
+  for (int i = threadIdx.x; i < tn; i += kThreads) {
+    uint32_t code = tcode[i];
+    if (tgone[i]) {
+      code = 0xFFFFFFFFu;
+    } else if (m.n_imp > 0) {
+      const uint64_t id = tids[i];
+      const uint64_t r1 = id / m.id_div, r2 = id % m.id_div;
+      for (int q = 0; q < m.n_imp; ++q)
+        if (m.imp_a[q][r1] == m.imp_b[q][r2]) code += m.imp_stride[q];
+    }
+    codes[t0 + i] = code;
   }
 }
 
@@ -700,6 +719,8 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
   FH_TRY(scr.alloc(&oids_tmp, (size_t)std::max<int64_t>(1, nmax)));
   FH_TRY(scr.alloc(&tot, 1));
   FH_TRY(scr.alloc(&bounds, (size_t)n_patterns + 1));
+  int64_t* tile_bounds = nullptr;
+  FH_TRY(scr.alloc(&tile_bounds, (size_t)((ceil_div(nmax, kMergeTile) + 1) * n_vars)));
   const int code_bits = bits_for(n_patterns - 1);
   int64_t base = 0;
   // Codes owned by variable v lie in [stride_v, stride_{v-1}); the output is
@@ -712,7 +733,12 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
       for (uint32_t c = code_lo; c <= code_hi; ++c) pattern_offsets[c] = base;
       continue;
     }
-    hipLaunchKernelGGL(k_pattern_codes, dim3((unsigned)ceil_div(n, kMergeTile)), dim3(kThreads), 0, s, m, v, codes);
+    const int64_t ntiles = ceil_div(n, kMergeTile);
+    hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div((ntiles + 1) * n_vars, kThreads)), dim3(kThreads), 0,
+                       s, m, v, ntiles, tile_bounds);
+    FH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_pattern_codes, dim3((unsigned)ntiles), dim3(kThreads), 0, s, m, v,
+                       (const int64_t*)tile_bounds, codes);
     FH_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_owner_flags, dim3(grid_for(n)), dim3(kThreads), 0, s, codes, n, flags);
     FH_HIP(hipGetLastError());
