@@ -112,6 +112,10 @@ int radix_sort_u64(uint64_t* keys, uint64_t* keys_tmp, uint64_t* vals, uint64_t*
 // Same for 32-bit keys with 64-bit values.
 int radix_sort_u32(uint32_t* keys, uint32_t* keys_tmp, uint64_t* vals, uint64_t* vals_tmp,
                    int64_t n, int begin_bit, int end_bit, hipStream_t s);
+// Sorts (in, vin) into (out, vout); in and vin are overwritten. An odd
+// number of 8-bit passes copies nothing.
+int radix_sort_u32_into(uint32_t* in, uint32_t* out, uint64_t* vin, uint64_t* vout, int64_t n, int begin_bit,
+                        int end_bit, hipStream_t s);
 
 inline int bits_for(uint64_t max_value) {
   int b = 0;

@@ -749,9 +749,9 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
     if (base + (int64_t)owners <= capacity && owners > 0) {
       oids = reinterpret_cast<uint64_t*>(ids + base);
       hipLaunchKernelGGL(k_owner_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, m.lists[v], codes, n, flags,
-                         ocodes, oids);
+                         ocodes_tmp, oids_tmp);
       FH_HIP(hipGetLastError());
-      FH_TRY(radix_sort_u32(ocodes, ocodes_tmp, oids, oids_tmp, owners, 0, code_bits, s));
+      FH_TRY(radix_sort_u32_into(ocodes_tmp, ocodes, oids_tmp, oids, owners, 0, code_bits, s));
       const uint32_t ncodes = code_hi - code_lo + 1;
       hipLaunchKernelGGL(k_code_bounds, dim3((unsigned)ceil_div(ncodes, kThreads)), dim3(kThreads), 0, s, ocodes,
                          (int64_t)owners, code_lo, code_hi, bounds);

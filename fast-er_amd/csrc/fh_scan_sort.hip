@@ -286,6 +286,54 @@ int radix_sort_impl(K* keys, K* keys_tmp, uint64_t* vals, uint64_t* vals_tmp, in
   return FH_OK;
 }
 
+// Sorts (in, vin) into (out, vout), using in and vin as the second buffer of
+// the passes: with an odd number of 8-bit passes the last one lands in out
+// and nothing is copied; with an even number the result is copied over.
+template <typename K>
+int radix_sort_into_impl(K* in, K* out, uint64_t* vin, uint64_t* vout, int64_t n, int begin_bit, int end_bit,
+                         hipStream_t s) {
+  if (n <= 0) return FH_OK;
+  const int passes = end_bit > begin_bit ? (int)ceil_div(end_bit - begin_bit, 8) : 0;
+  if (n == 1 || (passes & 1) == 0) {
+    if (n > 1) FH_TRY(radix_sort_impl<K>(in, out, vin, vout, n, begin_bit, end_bit, s));
+    FH_HIP(hipMemcpyAsync(out, in, (size_t)n * sizeof(K), hipMemcpyDeviceToDevice, s));
+    if (vin) FH_HIP(hipMemcpyAsync(vout, vin, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+    return FH_OK;
+  }
+  // An odd pass count: the passes alternate in -> out -> in -> ..., so the
+  // last one writes out.
+  const int64_t ntiles = ceil_div(n, kSortTile);
+  FH_REQUIRE(n < (int64_t)0xFFFFFFFF, "radix sort: n = %lld exceeds 2^32 - 1", (long long)n);
+  Scratch scr(s);
+  uint32_t* hist = nullptr;
+  FH_TRY(scr.alloc(&hist, (size_t)(256 * ntiles)));
+  K* src = in;
+  K* dst = out;
+  uint64_t* vsrc = vin;
+  uint64_t* vdst = vout;
+  for (int shift = begin_bit; shift < end_bit; shift += 8) {
+    hipLaunchKernelGGL(k_rs_hist<K>, dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, src, n, shift,
+                       ntiles, hist);
+    FH_HIP(hipGetLastError());
+    FH_TRY(exclusive_scan_impl<uint32_t>(hist, hist, 256 * ntiles, (uint32_t*)nullptr, s));
+    if (vin) {
+      hipLaunchKernelGGL((k_rs_scatter<K, true>), dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, src,
+                         vsrc, n, shift, ntiles, hist, dst, vdst);
+    } else {
+      hipLaunchKernelGGL((k_rs_scatter<K, false>), dim3((unsigned)ntiles), dim3(kSortThreads), 0, s, src,
+                         (const uint64_t*)nullptr, n, shift, ntiles, hist, dst, (uint64_t*)nullptr);
+    }
+    FH_HIP(hipGetLastError());
+    K* t = src;
+    src = dst;
+    dst = t;
+    uint64_t* vt = vsrc;
+    vsrc = vdst;
+    vdst = vt;
+  }
+  return FH_OK;
+}
+
 }  // namespace
 
 int exclusive_scan_u64(const uint64_t* in, uint64_t* out, int64_t n, uint64_t* total, hipStream_t s) {
@@ -303,6 +351,10 @@ int radix_sort_u32(uint32_t* keys, uint32_t* keys_tmp, uint64_t* vals, uint64_t*
                    int begin_bit, int end_bit, hipStream_t s) {
   return radix_sort_impl<uint32_t>(keys, keys_tmp, vals, vals_tmp, n, begin_bit, end_bit, s);
 }
+int radix_sort_u32_into(uint32_t* in, uint32_t* out, uint64_t* vin, uint64_t* vout, int64_t n, int begin_bit,
+                        int end_bit, hipStream_t s) {
+  return radix_sort_into_impl<uint32_t>(in, out, vin, vout, n, begin_bit, end_bit, s);
+}
 
 }  // namespace fh
 
@@ -316,4 +368,14 @@ extern "C" int fh_sort_u64(uint64_t* keys, int64_t n, int begin_bit, int end_bit
   uint64_t* tmp = nullptr;
   FH_TRY(scr.alloc(&tmp, (size_t)n));
   return fh::radix_sort_u64(keys, tmp, nullptr, nullptr, n, begin_bit, end_bit, s);
+}
+
+extern "C" int fh_sort_u64_into(uint64_t* in, uint64_t* out, int64_t n, int begin_bit, int end_bit,
+                                fh_stream_t stream) {
+  fh::clear_error();
+  FH_REQUIRE(n >= 0, "fh_sort_u64_into: n < 0");
+  FH_REQUIRE(begin_bit >= 0 && end_bit <= 64, "fh_sort_u64_into: bit range [%d, %d)", begin_bit, end_bit);
+  FH_REQUIRE(n == 0 || (in && out && in != out), "fh_sort_u64_into: in and out must be distinct buffers");
+  return fh::radix_sort_into_impl<uint64_t>(in, out, nullptr, nullptr, n, begin_bit, end_bit,
+                                            (hipStream_t)stream);
 }

@@ -145,12 +145,15 @@ def group_strings(chars: torch.Tensor, offsets: torch.Tensor, n: int):
 
 
 def sort_keys(keys: torch.Tensor, max_key: int) -> torch.Tensor:
-    """In-place ascending radix sort of level keys below ``max_key``."""
+    """Ascending radix sort of level keys below ``max_key``: returns the
+    sorted keys in a new tensor and leaves ``keys`` overwritten."""
     L = _lib.load()
     n = keys.numel()
-    if n > 1:
-        check(L.fh_sort_u64(ptr(keys), _i64(n), 0, bits_for(max_key), stream()), "fh_sort_u64")
-    return keys
+    if n <= 1:
+        return keys
+    out = torch.empty_like(keys)
+    check(L.fh_sort_u64_into(ptr(keys), ptr(out), _i64(n), 0, bits_for(max_key), stream()), "fh_sort_u64_into")
+    return out
 
 
 def split_levels(keys: torch.Tensor, n_levels: int):

@@ -63,7 +63,7 @@ def jaro_winkler_gpu(str1, str2, offset=0, p=0.1, lower_thr=0.88, upper_thr=0.94
     n1, n2 = len(o1) - 1, len(o2) - 1
     t = lambda a: torch.from_numpy(a if a.size else np.zeros(1, a.dtype)).to(dev)
     hits, _ = _ops.jw_levels(t(c1), t(o1), n1, t(c2), t(o2), n2, _lib.FH_MODE_LINK, p, lower_thr, upper_thr)
-    _ops.sort_keys(hits, _ops.max_pair_key(n1 * n2))
+    hits = _ops.sort_keys(hits, _ops.max_pair_key(n1 * n2))
     return [x + int(offset) for x in _split(hits, 3)]
 
 

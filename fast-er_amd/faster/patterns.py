@@ -146,8 +146,7 @@ def materialize_exact_only(all_equal_sorted, ids, offs, d, low):
     minus those already listed under a fuzzy pattern with exact digit d."""
     parts = [ids[offs[c]:offs[c + 1]] for c in range(low, len(offs) - 1) if c % low == d]
     listed = torch.cat(parts) if parts else ids[:0]
-    listed = listed.clone()
-    _ops.sort_keys(listed, int(listed.max().item()) + 1 if listed.numel() else 1)
+    listed = _ops.sort_keys(listed.clone(), int(listed.max().item()) + 1 if listed.numel() else 1)
     if listed.numel() == 0:
         return all_equal_sorted
     return all_equal_sorted[_ops.member_mask(all_equal_sorted, listed, False)]

@@ -249,6 +249,17 @@ int fh_exact_dedup(const int64_t* group, int64_t n, const int64_t* starts, const
 int fh_sort_u64(uint64_t* keys, int64_t n, int begin_bit, int end_bit, fh_stream_t stream);
 
 /*
+ * The same sort from `in` into `out` (distinct buffers of n keys); `in` is
+ * overwritten. With an odd number of 8-bit passes (key widths of 1-8, 17-24,
+ * 33-40 bits ...) nothing is copied, where the in-place form ends with a copy.
+ *
+ * Replaces the same cp.sort calls as fh_sort_u64 (faster/comparison.py:456,
+ * 487,588; faster/deduplication.py:577,614,689), whose unsorted input the
+ * reference drops after sorting.
+ */
+int fh_sort_u64_into(uint64_t* in, uint64_t* out, int64_t n, int begin_bit, int end_bit, fh_stream_t stream);
+
+/*
  * Stable split of a sorted level-key list into per-level id lists.
  *
  * The reference keeps one list per level (faster/comparison.py:498). ids

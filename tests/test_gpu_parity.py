@@ -167,12 +167,30 @@ def test_exact_link_load_balance(oracle):
 def test_sort_and_scan_primitives():
     import faster._ops as ops
 
+    from faster import _lib
+
+    L = _lib.load()
     rng = np.random.default_rng(9)
     for n in (0, 1, 5, 4095, 4096, 4097, 100_000, 1_500_000):
         x = rng.integers(0, 2 ** 46, n, dtype=np.int64)
         t = torch.from_numpy(x).cuda()
-        ops.sort_keys(t, 2 ** 46)
-        np.testing.assert_array_equal(t.cpu().numpy(), np.sort(x))
+        np.testing.assert_array_equal(ops.sort_keys(t, 2 ** 46).cpu().numpy(), np.sort(x))
+    # Both C entry points at odd and even pass counts (8, 16, 36 and 64 bits),
+    # with few distinct digits (bits 0..2 only) and with every digit in use.
+    for bits in (8, 16, 36, 64):
+        for hi in (8, 2 ** bits if bits < 64 else None):
+            x = rng.integers(0, hi, 50_000, dtype=np.uint64) if hi else rng.integers(0, 2 ** 63, 50_000).astype(
+                np.uint64) * np.uint64(2) + np.uint64(1)
+            want = np.sort(x)
+            print(f"sort bits={bits} hi={hi}", flush=True)
+            a = torch.from_numpy(x.view(np.int64)).cuda()
+            ops.check(L.fh_sort_u64(ops.ptr(a), ops._i64(a.numel()), 0, bits, ops.stream()), "fh_sort_u64")
+            np.testing.assert_array_equal(a.cpu().numpy().view(np.uint64), want)
+            a = torch.from_numpy(x.view(np.int64)).cuda()
+            out = torch.empty_like(a)
+            ops.check(L.fh_sort_u64_into(ops.ptr(a), ops.ptr(out), ops._i64(a.numel()), 0, bits, ops.stream()),
+                      "fh_sort_u64_into")
+            np.testing.assert_array_equal(out.cpu().numpy().view(np.uint64), want)
 
 
 def test_set_ops():
