@@ -90,6 +90,37 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const T* in, int64_
   if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = blk + tot;
 }
 
+// The whole scan in one workgroup, a tile at a time with a running carry:
+// one launch instead of three for the short scans (radix-sort digit counts
+// of small inputs, per-variable flags) whose cost is launch latency.
+constexpr int64_t kScanSingleMax = 16 * kScanTile;
+
+template <typename T>
+__global__ void __launch_bounds__(kScanThreads) k_scan_single(const T* in, int64_t n, T* out, T* total) {
+  T carry = 0;
+  for (int64_t t0 = 0; t0 < n; t0 += kScanTile) {
+    const int64_t base = t0 + (int64_t)threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const int64_t i = base + k;
+      v[k] = (i < n) ? in[i] : T(0);
+      sum += v[k];
+    }
+    T tot;
+    T run = block_exclusive_scan<T>(sum, &tot) + carry;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const int64_t i = base + k;
+      if (i < n) out[i] = run;
+      run += v[k];
+    }
+    carry += tot;
+  }
+  if (total && threadIdx.x == 0) *total = carry;
+}
+
 template <typename T>
 int exclusive_scan_impl(const T* in, T* out, int64_t n, T* total, hipStream_t s) {
   if (n <= 0) {
@@ -102,9 +133,8 @@ int exclusive_scan_impl(const T* in, T* out, int64_t n, T* total, hipStream_t s)
     return FH_OK;
   }
   const int64_t nb = ceil_div(n, kScanTile);
-  if (nb == 1) {
-    hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(kScanThreads), 0, s, in, n, (const T*)nullptr,
-                       out, total);
+  if (n <= kScanSingleMax) {
+    hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(kScanThreads), 0, s, in, n, out, total);
     FH_HIP(hipGetLastError());
     return FH_OK;
   }
