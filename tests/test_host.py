@@ -31,6 +31,12 @@ def test_library_exports_every_declared_symbol():
     assert set(decl) == set(L.EXPORTED)
     assert lib.fh_version() == 100
     assert lib.fh_last_error() == b""
+    # Argument checks run before any device work.
+    buf = ctypes.c_void_p(0x1000)
+    assert lib.fh_sort_u64_into(buf, buf, ctypes.c_int64(8), 0, 36, None) != 0
+    assert b"distinct buffers" in lib.fh_last_error()
+    assert lib.fh_sort_u64_into(None, None, ctypes.c_int64(-1), 0, 36, None) != 0
+    assert b"n < 0" in lib.fh_last_error()
 
 
 def test_ops_fail_loudly_without_gpu():
