@@ -369,12 +369,14 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
                                  d_planes_a, d_planes_b, s));
   }
   // The bound form's candidate pairs go to one buffer per round and are
-  // scored by a separate kernel, which finds a string-2 value's table column
-  // from its sorted index through the class starts and first tile of each
-  // class.
+  // scored by a separate kernel; the bit-parallel form scores its survivors
+  // in the kernel. Both find a string-2 value's table column from its sorted
+  // index through the class starts and first tile of each class.
   if (bound) {
     FH_TRY(scr.alloc(&d_cand, (size_t)cand_cap));
     FH_TRY(scr.alloc(&cand_count, 1));
+  }
+  {
     std::vector<int64_t> cls(2 * kNumClasses, 0);
     for (int c = 0; c < kNumClasses; ++c) cls[c] = start_b[c];
     for (int64_t t = (int64_t)pl.tiles.size() - 1; t >= 0; --t) cls[kNumClasses + pl.tiles[t].len] = t;

@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(256) k_eval_candidates(const JwArgs* __restric
     uint64_t x = 0, y = 0;
     if (e < n) {
       const uint64_t v = a.cand[e];
-      lvl = candidate_level(a, (int64_t)(v >> 32), (int64_t)(v & 0xFFFFFFFFull), x, y);
+      lvl = candidate_level<false>(a, (int64_t)(v >> 32), (int64_t)(v & 0xFFFFFFFFull), x, y);
     }
     emit_levels_wave(a, x, y, lvl);
   }
