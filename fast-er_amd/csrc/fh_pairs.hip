@@ -346,9 +346,14 @@ __global__ void __launch_bounds__(kThreads) k_tile_bounds(MergeArgs m, int v, in
   bounds[g] = b;
 }
 
+// With `tile_hist`, each tile also counts its owners per code:
+// tile_hist[(code - code_lo) * gridDim.x + tile], for k_code_scatter.
 __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, const int64_t* __restrict__ bounds,
-                                                            uint32_t* __restrict__ codes) {
+                                                            uint32_t* __restrict__ codes, uint32_t code_lo,
+                                                            int ncodes, uint32_t* __restrict__ tile_hist) {
+  static_assert(kThreads == 256, "one owner-count bin per thread");
   __shared__ uint64_t seg[kMergeLds];
+  __shared__ uint32_t th[256];
   __shared__ uint64_t tids[kMergeTile];   // the tile's pair ids, ascending
   __shared__ uint32_t tcode[kMergeTile];  // pattern code so far of each tile entry
   __shared__ uint8_t tgone[kMergeTile];   // 1: an earlier variable lists the pair
@@ -358,6 +363,7 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
   const int64_t t1 = min(t0 + (int64_t)kMergeTile, n);
   const int tn = (int)(t1 - t0);
   const int64_t per = (int64_t)gridDim.x + 1;
+  th[threadIdx.x] = 0;
   for (int i = threadIdx.x; i < tn; i += kThreads) {
     const uint64_t key = src[t0 + i];
     tids[i] = key >> 2;
@@ -447,6 +453,102 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
         if (m.imp_a[q][r1] == m.imp_b[q][r2]) code += m.imp_stride[q];
     }
     codes[t0 + i] = code;
+    if (tile_hist && code != 0xFFFFFFFFu) atomicAdd(&th[code - code_lo], 1u);
+  }
+  if (tile_hist) {
+    __syncthreads();
+    if ((int)threadIdx.x < ncodes) tile_hist[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = th[threadIdx.x];
+  }
+}
+
+// Owners of one list, ordered by code (stable: list order within a code),
+// written as (code, pair id) at their final positions: the one counting-sort
+// pass over the digit code - code_lo (fewer than 256 codes) that replaces
+// flagging, compacting and sorting the owners. A tile is the kMergeTile
+// entries of one k_pattern_codes tile, whose counts the scanned tile_hist
+// holds; wave w ranks entries [w * 256, (w + 1) * 256) of it, 64 at a time,
+// by eight ballots and an LDS atomic per code group, the tile's owners are
+// staged in LDS in code order and each code leaves as one run.
+__global__ void __launch_bounds__(kThreads) k_code_scatter(const uint32_t* __restrict__ codes,
+                                                           const uint64_t* __restrict__ keys, int64_t n,
+                                                           uint32_t code_lo, int ncodes,
+                                                           const uint32_t* __restrict__ hist_scanned,
+                                                           uint32_t* __restrict__ out_codes,
+                                                           uint64_t* __restrict__ out_ids) {
+  static_assert(kThreads == 256 && kMergeTile % kThreads == 0, "one code bin per thread, four waves");
+  constexpr int kWaveItems = kMergeTile / (kThreads / 64);
+  __shared__ uint32_t goff[256];
+  __shared__ uint32_t loff[256];
+  __shared__ uint32_t wcnt[kThreads / 64][256];
+  __shared__ uint32_t wtot[kThreads / 64];
+  __shared__ uint64_t sid[kMergeTile];
+  __shared__ uint32_t scode[kMergeTile];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  goff[tid] = tid < ncodes ? hist_scanned[(int64_t)tid * gridDim.x + blockIdx.x] : 0u;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; ++w) wcnt[w][tid] = 0;
+  __syncthreads();
+  const uint64_t lt = fhd::lanemask_lt();
+  const int64_t wbase = (int64_t)blockIdx.x * kMergeTile + (int64_t)wave * kWaveItems;
+  uint32_t code[kWaveItems / 64], rnk[kWaveItems / 64];
+#pragma unroll
+  for (int r = 0; r < kWaveItems / 64; ++r) {
+    const int64_t e = wbase + r * 64 + lane;
+    code[r] = e < n ? codes[e] : 0xFFFFFFFFu;
+    const bool valid = code[r] != 0xFFFFFFFFu;
+    const uint32_t d = valid ? code[r] - code_lo : 0u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    uint32_t old = 0;
+    if (valid && rank == 0) old = atomicAdd(&wcnt[wave][d], (uint32_t)__popcll(peers));
+    const int leader = valid ? __ffsll((unsigned long long)peers) - 1 : lane;
+    rnk[r] = (uint32_t)__shfl(old, leader, 64) + rank;
+  }
+  __syncthreads();
+  {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+      const uint32_t t = wcnt[w][tid];
+      wcnt[w][tid] = run;
+      run += t;
+    }
+    uint32_t inc = run;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t o = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += o;
+    }
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < wave; ++w) before += wtot[w];
+    loff[tid] = before + inc - run;
+  }
+  __syncthreads();
+  const int owners = (int)(wtot[0] + wtot[1] + wtot[2] + wtot[3]);
+#pragma unroll
+  for (int r = 0; r < kWaveItems / 64; ++r) {
+    if (code[r] != 0xFFFFFFFFu) {
+      const uint32_t d = code[r] - code_lo;
+      const uint32_t slot = loff[d] + wcnt[wave][d] + rnk[r];
+      sid[slot] = keys[wbase + r * 64 + lane] >> 2;
+      scode[slot] = code[r];
+    }
+  }
+  __syncthreads();
+  for (int j = tid; j < owners; j += kThreads) {
+    const uint32_t c = scode[j];
+    const uint32_t d = c - code_lo;
+    const uint32_t pos = goff[d] + ((uint32_t)j - loff[d]);
+    out_codes[pos] = c;
+    out_ids[pos] = sid[j];
   }
 }
 
@@ -721,6 +823,8 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
   FH_TRY(scr.alloc(&bounds, (size_t)n_patterns + 1));
   int64_t* tile_bounds = nullptr;
   FH_TRY(scr.alloc(&tile_bounds, (size_t)((ceil_div(nmax, kMergeTile) + 1) * n_vars)));
+  uint32_t* tile_hist = nullptr;  // owners per (code, tile), for the one-pass code sort
+  FH_TRY(scr.alloc(&tile_hist, (size_t)(256 * ceil_div(std::max<int64_t>(1, nmax), kMergeTile))));
   const int code_bits = bits_for(n_patterns - 1);
   int64_t base = 0;
   // Codes owned by variable v lie in [stride_v, stride_{v-1}); the output is
@@ -737,22 +841,36 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
     hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div((ntiles + 1) * n_vars, kThreads)), dim3(kThreads), 0,
                        s, m, v, ntiles, tile_bounds);
     FH_HIP(hipGetLastError());
+    const uint32_t ncodes = code_hi - code_lo + 1;
+    // Up to 256 codes per variable: the owners' counts per tile come out of
+    // the code kernel and one ranked scatter places them; more codes take
+    // the general path (flag, compact, radix sort by code).
+    const bool one_pass = ncodes <= 256;
     hipLaunchKernelGGL(k_pattern_codes, dim3((unsigned)ntiles), dim3(kThreads), 0, s, m, v,
-                       (const int64_t*)tile_bounds, codes);
+                       (const int64_t*)tile_bounds, codes, code_lo, (int)ncodes, one_pass ? tile_hist : nullptr);
     FH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_owner_flags, dim3(grid_for(n)), dim3(kThreads), 0, s, codes, n, flags);
-    FH_HIP(hipGetLastError());
-    FH_TRY(exclusive_scan_u32(flags, flags, n, tot, s));
+    if (one_pass) {
+      FH_TRY(exclusive_scan_u32(tile_hist, tile_hist, (int64_t)ncodes * ntiles, tot, s));
+    } else {
+      hipLaunchKernelGGL(k_owner_flags, dim3(grid_for(n)), dim3(kThreads), 0, s, codes, n, flags);
+      FH_HIP(hipGetLastError());
+      FH_TRY(exclusive_scan_u32(flags, flags, n, tot, s));
+    }
     uint32_t owners = 0;
     FH_HIP(hipMemcpyAsync(&owners, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     FH_HIP(hipStreamSynchronize(s));
     if (base + (int64_t)owners <= capacity && owners > 0) {
       oids = reinterpret_cast<uint64_t*>(ids + base);
-      hipLaunchKernelGGL(k_owner_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, m.lists[v], codes, n, flags,
-                         ocodes_tmp, oids_tmp);
-      FH_HIP(hipGetLastError());
-      FH_TRY(radix_sort_u32_into(ocodes_tmp, ocodes, oids_tmp, oids, owners, 0, code_bits, s));
-      const uint32_t ncodes = code_hi - code_lo + 1;
+      if (one_pass) {
+        hipLaunchKernelGGL(k_code_scatter, dim3((unsigned)ntiles), dim3(kThreads), 0, s, codes, m.lists[v], n,
+                           code_lo, (int)ncodes, (const uint32_t*)tile_hist, ocodes, oids);
+        FH_HIP(hipGetLastError());
+      } else {
+        hipLaunchKernelGGL(k_owner_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, m.lists[v], codes, n, flags,
+                           ocodes_tmp, oids_tmp);
+        FH_HIP(hipGetLastError());
+        FH_TRY(radix_sort_u32_into(ocodes_tmp, ocodes, oids_tmp, oids, owners, 0, code_bits, s));
+      }
       hipLaunchKernelGGL(k_code_bounds, dim3((unsigned)ceil_div(ncodes, kThreads)), dim3(kThreads), 0, s, ocodes,
                          (int64_t)owners, code_lo, code_hi, bounds);
       FH_HIP(hipGetLastError());

@@ -231,6 +231,29 @@ def test_comparison_fit_small(oracle):
         comp.fit()
 
 
+def test_comparison_fit_many_variables(oracle):
+    """Six fuzzy variables (729 patterns): the first variable owns 486 codes
+    and takes the general owner path (flag, compact, radix sort by code);
+    the others own at most 162 and take the one-pass ranked scatter."""
+    import faster
+
+    dfa, dfb, _ = _frames(700, 800, 5)
+    base = ["first_name", "last_name", "street"]
+    for df in (dfa, dfb):
+        for v in base:
+            df[v + "_r"] = df[v].str[::-1]
+    vf = base + [v + "_r" for v in base]
+    comp = faster.Comparison(dfa, dfb, vf, vf)
+    comp.fit()
+    per_var, indices, counts = oracle.comparison_fit([dfa[v].to_numpy() for v in vf], [dfb[v].to_numpy() for v in vf])
+    print(f"many variables: patterns={len(counts)} union={int(np.sum(counts[1:]))}", flush=True)
+    assert len(counts) == 3 ** 6
+    for v in range(6):
+        _eq_lists(comp.Levels[v], per_var[v], f"var {v}")
+    _eq_lists(comp.Indices, indices, "Indices")
+    np.testing.assert_array_equal(comp.Counts, counts)
+
+
 def test_deduplication_fit(oracle):
     import faster
     from faster.synthetic import dedup_table
