@@ -469,12 +469,20 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
 // holds; wave w ranks entries [w * 256, (w + 1) * 256) of it, 64 at a time,
 // by eight ballots and an LDS atomic per code group, the tile's owners are
 // staged in LDS in code order and each code leaves as one run.
+//
+// The list's output starts at state[0] (the owners of the variables already
+// placed, kept on the device); nothing is written when the owners would
+// pass `capacity` (the count still advances, so the caller sees the size).
 __global__ void __launch_bounds__(kThreads) k_code_scatter(const uint32_t* __restrict__ codes,
                                                            const uint64_t* __restrict__ keys, int64_t n,
                                                            uint32_t code_lo, int ncodes,
                                                            const uint32_t* __restrict__ hist_scanned,
+                                                           const uint32_t* __restrict__ list_owners,
+                                                           const uint64_t* __restrict__ state, int64_t capacity,
                                                            uint32_t* __restrict__ out_codes,
-                                                           uint64_t* __restrict__ out_ids) {
+                                                           int64_t* __restrict__ ids) {
+  if ((int64_t)state[0] + (int64_t)*list_owners > capacity) return;
+  uint64_t* __restrict__ out_ids = reinterpret_cast<uint64_t*>(ids + state[0]);
   static_assert(kThreads == 256 && kMergeTile % kThreads == 0, "one code bin per thread, four waves");
   constexpr int kWaveItems = kMergeTile / (kThreads / 64);
   __shared__ uint32_t goff[256];
@@ -588,6 +596,36 @@ __global__ void __launch_bounds__(kThreads) k_code_bounds(const uint32_t* __rest
       hi = mid;
   }
   offsets[c - code_lo] = lo;
+}
+
+// offsets[c] = state[0] + first index of code c among the list's owners
+// (out_codes, in code order), for c in [code_lo, code_hi]; state[0] alone
+// when the owners did not fit the output.
+__global__ void __launch_bounds__(kThreads) k_code_offsets(const uint32_t* __restrict__ out_codes,
+                                                           const uint32_t* __restrict__ list_owners,
+                                                           const uint64_t* __restrict__ state, int64_t capacity,
+                                                           uint32_t code_lo, uint32_t code_hi,
+                                                           int64_t* __restrict__ offsets) {
+  const int64_t c = (int64_t)code_lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > (int64_t)code_hi) return;
+  const int64_t base = (int64_t)state[0];
+  const int64_t n = (int64_t)*list_owners;
+  int64_t lo = 0;
+  if (base + n <= capacity) {
+    int64_t hi = n;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)out_codes[mid] < c)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+  }
+  offsets[c] = base + lo;
+}
+
+__global__ void k_advance_base(uint64_t* __restrict__ state, const uint32_t* __restrict__ list_owners) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) state[0] += *list_owners;
 }
 
 // ----------------------------- posteriors ----------------------------------
@@ -814,25 +852,84 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
   uint32_t *codes = nullptr, *flags = nullptr, *ocodes = nullptr, *ocodes_tmp = nullptr, *tot = nullptr;
   uint64_t *oids = nullptr, *oids_tmp = nullptr;
   int64_t* bounds = nullptr;
+  const int code_bits = bits_for(n_patterns - 1);
+  auto code_range = [&](int v, uint32_t* lo, uint32_t* hi) {
+    *lo = m.stride[v];
+    *hi = (v == 0 ? (uint32_t)n_patterns : m.stride[v - 1]) - 1;
+  };
+  // Up to 256 codes per variable: the owners' counts per tile come out of
+  // the code kernel and one ranked scatter places them, with the running
+  // output base and the code offsets kept on the device and read back once.
+  // More codes take the general path (flag, compact, radix sort by code),
+  // which reads each variable's owner count back.
+  bool one_pass = true;
+  for (int v = 0; v < n_vars; ++v) {
+    uint32_t lo, hi;
+    code_range(v, &lo, &hi);
+    one_pass = one_pass && hi - lo + 1 <= 256;
+  }
   FH_TRY(scr.alloc(&codes, (size_t)std::max<int64_t>(1, nmax)));
-  FH_TRY(scr.alloc(&flags, (size_t)std::max<int64_t>(1, nmax)));
   FH_TRY(scr.alloc(&ocodes, (size_t)std::max<int64_t>(1, nmax)));
-  FH_TRY(scr.alloc(&ocodes_tmp, (size_t)std::max<int64_t>(1, nmax)));
-  FH_TRY(scr.alloc(&oids_tmp, (size_t)std::max<int64_t>(1, nmax)));
   FH_TRY(scr.alloc(&tot, 1));
-  FH_TRY(scr.alloc(&bounds, (size_t)n_patterns + 1));
   int64_t* tile_bounds = nullptr;
   FH_TRY(scr.alloc(&tile_bounds, (size_t)((ceil_div(nmax, kMergeTile) + 1) * n_vars)));
   uint32_t* tile_hist = nullptr;  // owners per (code, tile), for the one-pass code sort
-  FH_TRY(scr.alloc(&tile_hist, (size_t)(256 * ceil_div(std::max<int64_t>(1, nmax), kMergeTile))));
-  const int code_bits = bits_for(n_patterns - 1);
+  if (one_pass) {
+    FH_TRY(scr.alloc(&tile_hist, (size_t)(256 * ceil_div(std::max<int64_t>(1, nmax), kMergeTile))));
+  } else {
+    FH_TRY(scr.alloc(&flags, (size_t)std::max<int64_t>(1, nmax)));
+    FH_TRY(scr.alloc(&ocodes_tmp, (size_t)std::max<int64_t>(1, nmax)));
+    FH_TRY(scr.alloc(&oids_tmp, (size_t)std::max<int64_t>(1, nmax)));
+    FH_TRY(scr.alloc(&bounds, (size_t)n_patterns + 1));
+  }
   int64_t base = 0;
+  if (one_pass) {
+    uint64_t* state = nullptr;  // [0] owners placed so far
+    int64_t* d_offsets = nullptr;
+    FH_TRY(scr.alloc(&state, 1));
+    FH_TRY(scr.alloc(&d_offsets, (size_t)n_patterns + 1));
+    FH_TRY(fill_u64(state, 0, 1, s));
+    for (int v = n_vars - 1; v >= 0; --v) {
+      const int64_t n = var_sizes[v];
+      uint32_t code_lo, code_hi;
+      code_range(v, &code_lo, &code_hi);
+      const uint32_t ncodes = code_hi - code_lo + 1;
+      const int64_t ntiles = std::max<int64_t>(1, ceil_div(n, kMergeTile));
+      if (n > 0) {
+        hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div((ntiles + 1) * n_vars, kThreads)), dim3(kThreads),
+                           0, s, m, v, ntiles, tile_bounds);
+        FH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_pattern_codes, dim3((unsigned)ntiles), dim3(kThreads), 0, s, m, v,
+                           (const int64_t*)tile_bounds, codes, code_lo, (int)ncodes, tile_hist);
+        FH_HIP(hipGetLastError());
+        FH_TRY(exclusive_scan_u32(tile_hist, tile_hist, (int64_t)ncodes * ntiles, tot, s));
+        hipLaunchKernelGGL(k_code_scatter, dim3((unsigned)ntiles), dim3(kThreads), 0, s, codes, m.lists[v], n,
+                           code_lo, (int)ncodes, (const uint32_t*)tile_hist, (const uint32_t*)tot,
+                           (const uint64_t*)state, capacity, ocodes, ids);
+        FH_HIP(hipGetLastError());
+      } else {
+        FH_TRY(fill_u32(tot, 0, 1, s));
+      }
+      hipLaunchKernelGGL(k_code_offsets, dim3((unsigned)ceil_div(ncodes, kThreads)), dim3(kThreads), 0, s, ocodes,
+                         (const uint32_t*)tot, (const uint64_t*)state, capacity, code_lo, code_hi, d_offsets);
+      FH_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_advance_base, dim3(1), dim3(64), 0, s, state, (const uint32_t*)tot);
+      FH_HIP(hipGetLastError());
+    }
+    uint64_t h_base = 0;
+    const uint32_t first = m.stride[n_vars - 1];
+    FH_HIP(hipMemcpyAsync(pattern_offsets + first, d_offsets + first, (size_t)(n_patterns - first) * sizeof(int64_t),
+                          hipMemcpyDeviceToHost, s));
+    FH_HIP(hipMemcpyAsync(&h_base, state, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+    FH_HIP(hipStreamSynchronize(s));
+    base = (int64_t)h_base;
+  } else {
   // Codes owned by variable v lie in [stride_v, stride_{v-1}); the output is
   // pattern-major, so variables are processed from the last to the first.
   for (int v = n_vars - 1; v >= 0; --v) {
     const int64_t n = var_sizes[v];
-    const uint32_t code_lo = m.stride[v];
-    const uint32_t code_hi = (v == 0 ? (uint32_t)n_patterns : m.stride[v - 1]) - 1;
+    uint32_t code_lo, code_hi;
+    code_range(v, &code_lo, &code_hi);
     if (n == 0) {
       for (uint32_t c = code_lo; c <= code_hi; ++c) pattern_offsets[c] = base;
       continue;
@@ -841,36 +938,22 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
     hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div((ntiles + 1) * n_vars, kThreads)), dim3(kThreads), 0,
                        s, m, v, ntiles, tile_bounds);
     FH_HIP(hipGetLastError());
-    const uint32_t ncodes = code_hi - code_lo + 1;
-    // Up to 256 codes per variable: the owners' counts per tile come out of
-    // the code kernel and one ranked scatter places them; more codes take
-    // the general path (flag, compact, radix sort by code).
-    const bool one_pass = ncodes <= 256;
     hipLaunchKernelGGL(k_pattern_codes, dim3((unsigned)ntiles), dim3(kThreads), 0, s, m, v,
-                       (const int64_t*)tile_bounds, codes, code_lo, (int)ncodes, one_pass ? tile_hist : nullptr);
+                       (const int64_t*)tile_bounds, codes, code_lo, 0, (uint32_t*)nullptr);
     FH_HIP(hipGetLastError());
-    if (one_pass) {
-      FH_TRY(exclusive_scan_u32(tile_hist, tile_hist, (int64_t)ncodes * ntiles, tot, s));
-    } else {
-      hipLaunchKernelGGL(k_owner_flags, dim3(grid_for(n)), dim3(kThreads), 0, s, codes, n, flags);
-      FH_HIP(hipGetLastError());
-      FH_TRY(exclusive_scan_u32(flags, flags, n, tot, s));
-    }
+    hipLaunchKernelGGL(k_owner_flags, dim3(grid_for(n)), dim3(kThreads), 0, s, codes, n, flags);
+    FH_HIP(hipGetLastError());
+    FH_TRY(exclusive_scan_u32(flags, flags, n, tot, s));
     uint32_t owners = 0;
     FH_HIP(hipMemcpyAsync(&owners, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     FH_HIP(hipStreamSynchronize(s));
     if (base + (int64_t)owners <= capacity && owners > 0) {
       oids = reinterpret_cast<uint64_t*>(ids + base);
-      if (one_pass) {
-        hipLaunchKernelGGL(k_code_scatter, dim3((unsigned)ntiles), dim3(kThreads), 0, s, codes, m.lists[v], n,
-                           code_lo, (int)ncodes, (const uint32_t*)tile_hist, ocodes, oids);
-        FH_HIP(hipGetLastError());
-      } else {
-        hipLaunchKernelGGL(k_owner_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, m.lists[v], codes, n, flags,
-                           ocodes_tmp, oids_tmp);
-        FH_HIP(hipGetLastError());
-        FH_TRY(radix_sort_u32_into(ocodes_tmp, ocodes, oids_tmp, oids, owners, 0, code_bits, s));
-      }
+      hipLaunchKernelGGL(k_owner_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, m.lists[v], codes, n, flags,
+                         ocodes_tmp, oids_tmp);
+      FH_HIP(hipGetLastError());
+      FH_TRY(radix_sort_u32_into(ocodes_tmp, ocodes, oids_tmp, oids, owners, 0, code_bits, s));
+      const uint32_t ncodes = code_hi - code_lo + 1;
       hipLaunchKernelGGL(k_code_bounds, dim3((unsigned)ceil_div(ncodes, kThreads)), dim3(kThreads), 0, s, ocodes,
                          (int64_t)owners, code_lo, code_hi, bounds);
       FH_HIP(hipGetLastError());
@@ -882,6 +965,7 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
       for (uint32_t c = code_lo; c <= code_hi; ++c) pattern_offsets[c] = base;
     }
     base += owners;
+  }
   }
   pattern_offsets[0] = 0;
   pattern_offsets[n_patterns] = base;
