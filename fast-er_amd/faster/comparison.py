@@ -168,11 +168,17 @@ class Comparison():
             rowid_dev = [(d[0], b) for d, b in zip(dev, dev_b)]
             rowid_host = [(g.group_a, g.group_b) for g in exact]
 
+            def equal_pairs(S):
+                if len(S) == 1:
+                    ga, sb, rb = dev[S[0]]
+                    return exact_keys_link(ga, sb, rb, exact[S[0]].n_groups, n_b)
+                ja, jb = patterns.joint_ids([exact[q].group_a for q in S], [exact[q].group_b for q in S])
+                g = group_exact_link(ja, jb)
+                ga, sb, rb = g.to_device()
+                return exact_keys_link(ga, sb, rb, g.n_groups, n_b)
+
             def mat(d, ids, offs):
-                if len(exact) != 1:
-                    raise NotImplementedError("materialising exact-only patterns needs a single exact variable")
-                eq = _split(exact_keys_link(dev[0][0], dev[0][1], dev[0][2], exact[0].n_groups, n_b), 2)[0]
-                return patterns.materialize_exact_only(eq, ids, offs, d, 2)
+                return patterns.materialize_exact_only(d, len(exact), ids, offs, rowid_dev, n_b, equal_pairs)
 
             ids, offs, self.Indices, self._Counts = patterns.merge_with_rowids(
                 keys, lv_f, rowid_dev, rowid_host, n_b, False, n_a * n_b, mat)

@@ -110,12 +110,16 @@ class Deduplication():
             rowid_dev = [(d[0], d[0]) for d in dev]
             rowid_host = [(g.group, g.group) for g in exact]
 
+            def equal_pairs(S):
+                if len(S) == 1:
+                    grp, st, rows = dev[S[0]]
+                    return _ops.exact_dedup(grp, st, rows, exact[S[0]].n_groups)
+                g = group_exact_dedup(patterns.joint_ids([exact[q].group for q in S]))
+                grp, st, rows = g.to_device()
+                return _ops.exact_dedup(grp, st, rows, g.n_groups)
+
             def mat(d, ids, offs):
-                if len(exact) != 1:
-                    raise NotImplementedError("materialising exact-only patterns needs a single exact variable")
-                grp, st, rows = dev[0]
-                eq = _split(_ops.exact_dedup(grp, st, rows, exact[0].n_groups), 2)[0]
-                return patterns.materialize_exact_only(eq, ids, offs, d, 2)
+                return patterns.materialize_exact_only(d, len(exact), ids, offs, rowid_dev, n, equal_pairs)
 
             ids, offs, self.Indices, self._Counts = patterns.merge_with_rowids(
                 keys, lv_f, rowid_dev, rowid_host, n, True, n * (n + 1) // 2, mat)

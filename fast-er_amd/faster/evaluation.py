@@ -1,10 +1,19 @@
 """Evaluation of a fitted model (drop-in for faster/evaluation.py, fast-er 0.3.1).
 
-Host arithmetic over the pattern histogram (at most a few thousand bins).
-``FNR`` keeps the reference's operator precedence,
-``sum(Ksi * (Ksi < S) * Counts) / Lambda * sum(Counts)``
-(faster/evaluation.py:43), so results match the reference bit for bit;
-``FNR_rate`` gives the rate the docstring describes.
+Host arithmetic over the pattern histogram (at most a few thousand bins),
+evaluated on the objects the caller passed in, as the reference does:
+
+* a threshold that selects no pair makes ``FDR`` a NumPy 0/0, i.e. ``nan``
+  (with NumPy's RuntimeWarning), and ``Lambda == 0`` makes ``FNR`` ``inf``
+  or ``nan`` the same way (faster/evaluation.py:22-47 divides NumPy scalars,
+  which never raises);
+* any exception while evaluating (for example ``Ksi`` given as a plain list,
+  which does not support ``1 - Ksi``) returns ``None`` (the bare ``except``
+  of faster/evaluation.py:31-32,44-45);
+* ``FNR`` keeps the reference's operator precedence,
+  ``sum(Ksi * (Ksi < S) * Counts) / Lambda * sum(Counts)``
+  (faster/evaluation.py:43); ``FNR_rate`` gives the rate the docstring
+  describes.
 """
 from __future__ import annotations
 
@@ -24,48 +33,57 @@ class Evaluation():
         self.Ksi = Ksi
         self.Counts = Counts
 
+    def _selected(self, S):
+        """(pairs with Ksi >= S weighted by Counts, the same below S)."""
+        above = (self.Ksi >= S) * self.Counts
+        below = (self.Ksi < S) * self.Counts
+        return above, below
+
     def FDR(self, S: float):
-        """Share of non-matches among pairs with ``Ksi >= S`` (None when no pair qualifies)."""
-        ksi = np.asarray(self.Ksi, dtype=np.float64)
-        counts = np.asarray(self.Counts)
-        den = np.sum((ksi >= S) * counts)
-        if den == 0:
+        """Share of non-matches among pairs with ``Ksi >= S``; ``nan`` when no
+        pair qualifies, ``None`` when the inputs cannot be evaluated."""
+        try:
+            above, _ = self._selected(S)
+            false_matches = np.sum((1 - self.Ksi) * above)
+            return false_matches / np.sum(above)
+        except Exception:
             return None
-        return np.sum((1 - ksi) * (ksi >= S) * counts) / den
 
     def FNR(self, S: float):
-        """The reference's expression for the false-negative rate."""
-        ksi = np.asarray(self.Ksi, dtype=np.float64)
-        counts = np.asarray(self.Counts)
-        if self.Lambda == 0:
+        """The reference's expression for the false-negative rate (its
+        operator precedence kept); ``None`` when the inputs cannot be
+        evaluated."""
+        try:
+            _, below = self._selected(S)
+            missed = np.sum(self.Ksi * below)
+            return missed / self.Lambda * np.sum(self.Counts)
+        except Exception:
             return None
-        return np.sum(ksi * (ksi < S) * counts) / self.Lambda * np.sum(counts)
 
     def FNR_rate(self, S: float):
         """Share of expected matches with ``Ksi < S``:
         ``sum(Ksi * (Ksi < S) * Counts) / (Lambda * sum(Counts))``."""
-        ksi = np.asarray(self.Ksi, dtype=np.float64)
-        counts = np.asarray(self.Counts)
-        den = self.Lambda * np.sum(counts)
-        if den == 0:
+        try:
+            _, below = self._selected(S)
+            expected = np.float64(self.Lambda) * np.sum(self.Counts)
+            return np.sum(self.Ksi * below) / expected
+        except Exception:
             return None
-        return np.sum(ksi * (ksi < S) * counts) / den
 
     def Frontier(self):
         """Plots FDR against FNR for thresholds 0, 0.001, ..., 1."""
         import matplotlib.pyplot as plt
 
-        plt.plot([self.FDR(s / 1000) for s in range(1001)], [self.FNR(s / 1000) for s in range(1001)], ".-")
+        grid = [s / 1000 for s in range(1001)]
+        plt.plot([self.FDR(s) for s in grid], [self.FNR(s) for s in grid], ".-")
         plt.xlabel("False Discovery Rate (FDR)")
         plt.ylabel("False Negative Rate (FNR)")
         plt.show()
 
     def Optimal_Threshold(self, Alpha: float):
-        """Threshold in steps of 0.001 minimising ``FDR + Alpha * FNR``."""
-        vals = []
-        for s in range(1001):
-            fdr = self.FDR(s / 1000)
-            fnr = self.FNR(s / 1000)
-            v = np.nan if fdr is None or fnr is None else fdr + Alpha * fnr
-            vals.append(v)
-        return np.argmin(np.nan_to_num(np.array(vals, dtype=np.float64), nan=1 + Alpha)) / 1000
+        """Threshold in steps of 0.001 minimising ``FDR + Alpha * FNR``;
+        ``nan`` objectives count as ``1 + Alpha`` and infinite ones as the
+        largest float (``np.nan_to_num``). A ``None`` from ``FDR`` or ``FNR``
+        raises ``TypeError``, as in the reference."""
+        objective = [self.FDR(s / 1000) + Alpha * self.FNR(s / 1000) for s in range(1001)]
+        return np.argmin(np.nan_to_num(objective, nan=1 + Alpha)) / 1000

@@ -126,9 +126,9 @@ def merge_with_rowids(keys, levels, rowid_dev, rowid_host, id_div, dedup, total_
     return ids, offs, indices, counts
 
 
-ROWID_EXACT_MIN_PAIRS = 1 << 31
-"""Exact variables whose equal-pair list would exceed this many pairs are
-kept as row ids ('auto' mode)."""
+ROWID_EXACT_MIN_PAIRS = (1 << 32) - 2
+"""Exact variables whose equal-pair list would exceed this many pairs (the
+longest list the pattern merge accepts) are kept as row ids in 'auto' mode."""
 
 
 def choose_exact_mode(mode, n_fuzzy, exact_pair_totals):
@@ -141,12 +141,50 @@ def choose_exact_mode(mode, n_fuzzy, exact_pair_totals):
     return mode
 
 
-def materialize_exact_only(all_equal_sorted, ids, offs, d, low):
-    """Pairs of exact-only pattern d for one exact variable: every equal pair
-    minus those already listed under a fuzzy pattern with exact digit d."""
+def exact_digit_vars(d, k):
+    """Exact variables with digit 1 in the exact part ``d`` of a pattern code
+    (first exact variable most significant)."""
+    return [q for q in range(k) if (d >> (k - 1 - q)) & 1]
+
+
+def joint_ids(cols_a, cols_b=None):
+    """Joint value ids of several id columns (equal ids on every column <=>
+    equal joint id); with ``cols_b`` the ids are shared by both sides."""
+    if cols_b is None:
+        return np.asarray(_joint_ids(cols_a), dtype=np.int64)
+    na = len(cols_a[0])
+    j = np.asarray(_joint_ids([np.concatenate([a, b]) for a, b in zip(cols_a, cols_b)]), dtype=np.int64)
+    return j[:na], j[na:]
+
+
+def materialize_exact_only(d, k, ids, offs, rowid_dev, id_div, equal_pairs):
+    """Pair ids (ascending, on the device) of the exact-only pattern whose
+    exact digits are ``d`` and whose fuzzy digits are all 0.
+
+    ``equal_pairs(S)`` returns the ascending level keys of every pair equal on
+    all exact variables in S (the reference's exact lists,
+    faster/comparison.py:500-600 and faster/deduplication.py:628-701, over the
+    joint values of S). Pairs that also agree on an exact variable outside S
+    belong to another pattern and are dropped; pairs listed under a pattern
+    with a fuzzy agreement and exact digits ``d`` are removed last, which
+    leaves exactly the pairs of the reference's setdiff cascade
+    (faster/comparison.py:700-725) for this pattern."""
+    low = 2 ** k
+    S = exact_digit_vars(d, k)
+    keys = equal_pairs(S)
+    pair = keys >> 2
+    if pair.numel():
+        r1 = pair // int(id_div)
+        r2 = pair % int(id_div)
+        keep = torch.ones(pair.numel(), dtype=torch.bool, device=pair.device)
+        for q in range(k):
+            if q not in S:
+                a, b = rowid_dev[q]
+                keep &= a[r1] != b[r2]
+        pair = pair[keep]
     parts = [ids[offs[c]:offs[c + 1]] for c in range(low, len(offs) - 1) if c % low == d]
     listed = torch.cat(parts) if parts else ids[:0]
-    listed = _ops.sort_keys(listed.clone(), int(listed.max().item()) + 1 if listed.numel() else 1)
-    if listed.numel() == 0:
-        return all_equal_sorted
-    return all_equal_sorted[_ops.member_mask(all_equal_sorted, listed, False)]
+    if listed.numel() == 0 or pair.numel() == 0:
+        return pair
+    listed = _ops.sort_keys(listed.clone(), int(listed.max().item()) + 1)
+    return pair[_ops.member_mask(pair, listed, False)]

@@ -345,9 +345,10 @@ def test_linkage_pattern_zero_compat():
 
 @pytest.mark.parametrize("which", ["dedup", "link"])
 def test_rowid_exact_mode(oracle, which):
-    """Exact variables kept as row ids: Counts equal the oracle's (listed)
-    Counts, fuzzy-pattern lists are identical, and the lazy exact-only lists
-    materialise to the oracle's lists."""
+    """Two exact variables kept as row ids: Counts equal the oracle's (listed)
+    Counts, fuzzy-pattern lists are identical, every lazy exact-only list
+    materialises to the oracle's list, and Linkage.transform links the pairs
+    of an exact-only pattern."""
     import faster
     from faster.patterns import RowIdPairs
     from faster.synthetic import dedup_table, linkage_tables
@@ -363,24 +364,32 @@ def test_rowid_exact_mode(oracle, which):
                                                       [df["birth_year"].to_numpy(), df["flag"].to_numpy()])
     else:
         dfa, dfb, _ = linkage_tables(700, 800, seed=22)
-        dfa["house_number"] = dfa["house_number"] % 5
-        dfb["house_number"] = dfb["house_number"] % 5
-        obj = faster.Comparison(dfa, dfb, vf, vf, ["house_number"], ["house_number"])
+        for d, k in ((dfa, 1), (dfb, 2)):
+            d["house_number"] = d["house_number"] % 5
+            d["flag"] = (np.arange(len(d)) * k % 3).astype(np.int64)
+        ve = ["house_number", "flag"]
+        obj = faster.Comparison(dfa, dfb, vf, vf, ve, ve)
         obj.fit(Exact_Mode="rowid")
         _, indices, counts = oracle.comparison_fit([dfa[v].to_numpy() for v in vf], [dfb[v].to_numpy() for v in vf],
-                                                   [dfa["house_number"].to_numpy()], [dfb["house_number"].to_numpy()])
+                                                   [dfa[v].to_numpy() for v in ve], [dfb[v].to_numpy() for v in ve])
     assert obj.Exact_Mode == "rowid"
     np.testing.assert_array_equal(obj.Counts, counts)
-    lazy = 0
+    lazy = []
     for k, (g, r) in enumerate(zip(obj.Indices, indices)):
         if isinstance(g, RowIdPairs):
-            lazy += 1
+            lazy.append(k + 1)
             assert g.numel() == len(r)
-            if which == "link":
-                np.testing.assert_array_equal(_np(g.materialize()), np.asarray(r, dtype=np.int64))
+            np.testing.assert_array_equal(_np(g.materialize()), np.asarray(r, dtype=np.int64),
+                                          err_msg=f"exact-only pattern {k + 1}")
         else:
             np.testing.assert_array_equal(_np(g), np.asarray(r, dtype=np.int64), err_msg=f"pattern {k + 1}")
-    assert lazy == (3 if which == "dedup" else 1)
+    assert lazy == [1, 2, 3]
+    if which == "link":
+        ksi = np.zeros(len(counts))
+        ksi[3] = 1.0  # exact digits (1, 1), both names disagree
+        out = faster.Linkage(dfa, dfb, obj.Indices, ksi).transform(0.85, Compat=False)
+        got = np.sort(out["Index_A"].to_numpy() * len(dfb) + out["Index_B"].to_numpy())
+        np.testing.assert_array_equal(got, np.asarray(indices[2], dtype=np.int64))
 
 
 def test_blocked_linkage_matches_per_block_loop(oracle):

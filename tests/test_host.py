@@ -126,17 +126,39 @@ def test_class_validation_and_errors():
 
 
 def test_evaluation_host_math():
+    """Values worked out by hand for Ksi = [0.01, 0.5, 0.99], Counts = [100,
+    10, 5], Lambda = 0.05 (faster/evaluation.py:22-67): the expected matches
+    below 0.9 are 0.01 * 100 + 0.5 * 10 = 6, so the reference's FNR is
+    6 / 0.05 * 115 = 13800 (division before the product) and the rate is
+    6 / (0.05 * 115). A threshold above every Ksi is a NumPy 0/0 (nan, no
+    exception), Lambda = 0 gives inf (or nan for 0/0), and a list Ksi makes
+    the reference's bare except return None."""
+    import warnings
+
     import faster
 
     ksi = np.array([0.01, 0.5, 0.99])
     counts = np.array([100, 10, 5])
     ev = faster.Evaluation(0.05, ksi, counts)
-    assert ev.FDR(0.9) == pytest.approx(0.01)
-    assert ev.FNR(0.9) == pytest.approx(np.sum(ksi * (ksi < 0.9) * counts) / 0.05 * counts.sum())
-    assert ev.FNR_rate(0.9) == pytest.approx(np.sum(ksi * (ksi < 0.9) * counts) / (0.05 * counts.sum()))
-    assert ev.FDR(1.1) is None
-    t = ev.Optimal_Threshold(1.0)
-    assert 0.0 <= t <= 1.0
+    assert ev.FDR(0.9) == pytest.approx(0.05 / 5)
+    assert ev.FDR(0.0) == pytest.approx((0.99 * 100 + 0.5 * 10 + 0.01 * 5) / 115)
+    assert ev.FNR(0.9) == pytest.approx(13800.0)
+    assert ev.FNR(0.0) == 0.0
+    assert ev.FNR_rate(0.9) == pytest.approx(6 / 5.75)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", RuntimeWarning)
+        fdr_empty = ev.FDR(1.1)
+        assert isinstance(fdr_empty, np.floating) and np.isnan(fdr_empty)
+        z = faster.Evaluation(0.0, ksi, counts)
+        assert np.isinf(z.FNR(0.9))
+        assert np.isnan(z.FNR(0.0))
+        # FDR + FNR is smallest at S = 0 (FNR is 0 there and >= 2300 above
+        # 0.01); nan objectives above 0.99 count as 1 + Alpha.
+        assert ev.Optimal_Threshold(1.0) == 0.0
+    as_list = faster.Evaluation(0.05, [0.01, 0.5, 0.99], [100, 10, 5])
+    assert as_list.FDR(0.5) is None and as_list.FNR(0.5) is None
+    with pytest.raises(TypeError):
+        as_list.Optimal_Threshold(1.0)
 
 
 def test_synthetic_deterministic():
