@@ -6,6 +6,7 @@
 // the E-step gives each thread a strided set of patterns, the M-step sums are
 // wave reductions (64-lane shuffles) combined through LDS, and the stopping
 // test (max |delta pi| < tol, faster/estimation.py:146) is evaluated in LDS.
+#include <mutex>
 #include <vector>
 
 #include "fh_common.h"
@@ -188,6 +189,72 @@ extern "C" int fh_em_fit(int n_vars, const int32_t* levels, const int64_t* count
   FH_TRY(scr.alloc(&wn, (size_t)sh.n_patterns));
   hipLaunchKernelGGL(k_em, dim3(1), dim3(kThreads), 0, s, sh, counts, params, tol, max_iter, ksi, wk, wn, status);
   FH_HIP(hipGetLastError());
+  int hs[2] = {0, 0};
+  FH_HIP(hipMemcpyAsync(hs, status, sizeof(hs), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  if (iterations) *iterations = hs[0];
+  if (converged) *converged = hs[1];
+  return FH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// EM without allocation: the per-pattern weights and the status words live in
+// a file-scope device array. Its host-side name is only a stand-in for the
+// symbol, so the device address is looked up once per device with
+// hipGetSymbolAddress and kept here; kernels get that address.
+// ---------------------------------------------------------------------------
+namespace fh {
+namespace {
+
+constexpr int kNoAllocMaxPatterns = 1 << 16;
+constexpr int kNoAllocMaxDevices = 64;
+// wk[kNoAllocMaxPatterns], wn[kNoAllocMaxPatterns], then two int status words.
+constexpr size_t kNoAllocDoubles = 2 * (size_t)kNoAllocMaxPatterns + 1;
+
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+int scratch_for_current_device(double** out) {
+  int dev = -1;
+  FH_HIP(hipGetDevice(&dev));
+  FH_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, "fh_em_fit_noalloc: device %d beyond %d", dev,
+             kNoAllocMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    FH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FH_REQUIRE(p != nullptr, "fh_em_fit_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return FH_OK;
+}
+
+}  // namespace
+}  // namespace fh
+
+extern "C" int fh_em_fit_noalloc(int n_vars, const int32_t* levels, const int64_t* counts, double* params,
+                                 double tol, int max_iter, double* ksi, int* iterations, int* converged,
+                                 fh_stream_t stream) {
+  using namespace fh;
+  clear_error();
+  EmShape sh;
+  FH_TRY(make_shape(n_vars, levels, &sh));
+  FH_REQUIRE(max_iter >= 0, "fh_em_fit_noalloc: max_iter < 0");
+  FH_REQUIRE(sh.n_patterns <= kNoAllocMaxPatterns,
+             "fh_em_fit_noalloc: %d patterns exceed the scratch limit of %d (use fh_em_fit)", sh.n_patterns,
+             kNoAllocMaxPatterns);
+  double* base = nullptr;
+  FH_TRY(scratch_for_current_device(&base));
+  double* wk = base;
+  double* wn = base + kNoAllocMaxPatterns;
+  int* status = reinterpret_cast<int*>(base + 2 * (size_t)kNoAllocMaxPatterns);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_em, dim3(1), dim3(kThreads), 0, s, sh, counts, params, tol, max_iter, ksi, wk, wn, status);
+  FH_HIP(hipGetLastError());
+  if (iterations == nullptr && converged == nullptr) return FH_OK;
   int hs[2] = {0, 0};
   FH_HIP(hipMemcpyAsync(hs, status, sizeof(hs), hipMemcpyDeviceToHost, s));
   FH_HIP(hipStreamSynchronize(s));

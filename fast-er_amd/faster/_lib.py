@@ -81,6 +81,8 @@ _SIGS = {
                                        ctypes.POINTER(_i64), _vp]),
     "fh_em_fit": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _f64, _i32, _vp,
                          ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp]),
+    "fh_em_fit_noalloc": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _f64, _i32, _vp,
+                                 ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp]),
     "fh_match_probability": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _vp]),
     "fh_pair_posteriors": (_i32, [_vp, _i32, _vp, _vp, _i64, _vp]),
     "fh_fs_loglik_fwd": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _vp, _vp, _vp]),

@@ -333,6 +333,24 @@ int fh_em_fit(int n_vars, const int32_t* levels, const int64_t* counts, double* 
               double tol, int max_iter, double* ksi, int* iterations, int* converged,
               fh_stream_t stream);
 
+/*
+ * fh_em_fit without any allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops that re-fit EM on updated histograms, harnesses that
+ * forbid hipMalloc). Arguments, results and kernels are those of fh_em_fit;
+ * the per-pattern weights and the status words live in a library-owned
+ * device array of 1 MiB per GPU (held until the process ends), so the
+ * histogram may have at most 65,536 patterns (FH_ERR_ARG beyond). The call
+ * allocates nothing. With iterations and converged both null it also reads
+ * nothing back (no synchronisation); otherwise it synchronises the stream.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (several streams or host threads) race and must be serialised by the
+ * caller.
+ */
+int fh_em_fit_noalloc(int n_vars, const int32_t* levels, const int64_t* counts, double* params,
+                      double tol, int max_iter, double* ksi, int* iterations, int* converged,
+                      fh_stream_t stream);
+
 /* Match probability of every pattern under `params` (layout as above);
  * faster/estimation.py:72-96 in FP64. */
 int fh_match_probability(int n_vars, const int32_t* levels, const double* params, double* ksi,
