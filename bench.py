@@ -130,6 +130,9 @@ def main():
                          "MFMA bound with exact scoring of the candidates (bound)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-blocked", action="store_true",
+                    help="link workload: skip the batched-block section (8M x 8M in Zipf blocks)")
+    ap.add_argument("--blocked-n", type=int, default=8_000_000, help="records per side of the batched-block section")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline duration")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -143,7 +146,7 @@ def main():
     import faster
     from faster import _ops
     from faster.comparison import exact_keys_link, fuzzy_keys_link
-    from faster.encode import group_exact_link, group_strings_device
+    from faster.encode import group_exact_link_device, group_packed
     from faster.synthetic import STRING_VARS, linkage_tables
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
@@ -173,29 +176,21 @@ def main():
     t0 = time.time()
     dfa, dfb, _ = linkage_tables(n, n, seed=args.seed)
     t_gen = time.time() - t0
-    log(f"generated in {t_gen:.1f} s; encoding columns (host packing, device grouping)")
+    vf = list(STRING_VARS)
+    comp0 = faster.Comparison(dfa, dfb, vf, vf, ["house_number"], ["house_number"])
+    log(f"generated in {t_gen:.1f} s; packing columns on the host and copying them to the device")
     t0 = time.time()
-    dgroups = [(group_strings_device(dfa[v].to_numpy()), group_strings_device(dfb[v].to_numpy()))
-               for v in STRING_VARS]
+    inputs = comp0.device_inputs()
     torch.cuda.synchronize()
-    t_enc_str = time.time() - t0
-    t0 = time.time()
-    ex = group_exact_link(dfa["house_number"].to_numpy(), dfb["house_number"].to_numpy())
-    ga_ex, sb_ex, rb_ex = ex.to_device()
-    torch.cuda.synchronize()
-    t_enc_ex = time.time() - t0
-    t_enc = t_enc_str + t_enc_ex
-    # Unique values on the host, for the CPU baseline only.
-    groups = [tuple(_HostGroups(g.chars.cpu().numpy(), g.offsets.cpu().numpy(), g.n_unique) for g in pair)
-              for pair in dgroups]
-    log(f"encoded in {t_enc:.2f} s (strings {t_enc_str:.2f} s, house number {t_enc_ex:.2f} s); unique per side: "
-        + ", ".join(f"{a.n_unique}/{b.n_unique}" for a, b in dgroups))
+    t_prep = time.time() - t0
+    log(f"host packing + copy {t_prep:.2f} s")
 
-    def step(stats=None):
-        keys = [fuzzy_keys_link(a, b, 0.1, 0.88, 0.94, stats) for a, b in dgroups]
-        keys.append(exact_keys_link(ga_ex, sb_ex, rb_ex, ex.n_groups, n))
-        ids, offs, n_union = _ops.pattern_merge(keys, [3, 3, 3, 2])
-        return keys, ids, offs, n_union
+    def step(stats_out=None):
+        c = faster.Comparison(dfa, dfb, vf, vf, ["house_number"], ["house_number"])
+        c.fit_device(inputs, 0.1, 0.88, 0.94, Exact_Mode="list")
+        if stats_out is not None:
+            stats_out.extend(c.Stats)
+        return c
 
     for w in range(args.warmup):
         log(f"warmup step {w + 1}/{args.warmup}")
@@ -203,16 +198,19 @@ def main():
         torch.cuda.synchronize()
     times = []
     stats_all = []
+    comp = None
     for k in range(args.steps):
         st = []
+        del comp
+        comp = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        keys, ids, offs, n_union = step(st)
+        comp = step(st)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         times.append(dt)
         stats_all.append(st)
-        log(f"step {k + 1}/{args.steps}: {dt * 1e3:.1f} ms")
+        log(f"step {k + 1}/{args.steps}: {dt * 1e3:.1f} ms (Comparison.fit device stage incl. grouping)")
     ms = 1e3 * sum(times) / len(times)
     comps = float(n) * float(n) * 3
     value = comps / (ms / 1e3)
@@ -222,8 +220,32 @@ def main():
     roof = comparison_roofline(stats_all, kname, measured_traffic(f"link_{args.method}", n))
     kern_ms = roof["kernel_ms"]
     st0 = stats_all[-1]
-    counts = np.concatenate([[n * n - int(n_union)], np.diff(np.asarray(offs))[1:]]).astype(np.int64)
-    level_sizes = [int(k.numel()) for k in keys]
+    counts = comp.Counts
+    level_sizes = [int(k.numel()) for k in comp._var_keys]
+    n_union = int(comp._pattern_ids.numel())
+    del comp
+
+    # The step from columns already grouped on the device (grouping outside).
+    dgroups = [(group_packed(a), group_packed(b)) for a, b in inputs.fuzzy]
+    ex = group_exact_link_device(inputs.exact[0], n)
+    torch.cuda.synchronize()
+
+    def step_grouped():
+        keys = [fuzzy_keys_link(a, b, 0.1, 0.88, 0.94) for a, b in dgroups]
+        keys.append(exact_keys_link(ex.group_a, ex.starts_b, ex.rows_b, ex.n_groups, n))
+        return _ops.pattern_merge(keys, [3, 3, 3, 2])
+
+    step_grouped()
+    g_times = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, offs_g, _ = step_grouped()
+        torch.cuda.synchronize()
+        g_times.append(time.perf_counter() - t0)
+    counts_g = np.concatenate([[n * n - int(offs_g[-1])], np.diff(np.asarray(offs_g))[1:]]).astype(np.int64)
+    log(f"step from grouped columns: {1e3 * np.mean(g_times):.1f} ms; counts equal: {np.array_equal(counts_g, counts)}")
+    del dgroups, ex
 
     # EM on the histogram (reported, not part of value).
     torch.cuda.synchronize()
@@ -236,10 +258,18 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        groups = [tuple(_HostGroups(g.chars.cpu().numpy(), g.offsets.cpu().numpy(), g.n_unique)
+                        for g in (group_packed(a), group_packed(b))) for a, b in inputs.fuzzy]
         cpu = cpu_baseline(groups, n, args.cpu_seconds)
+    del inputs
+    torch.cuda.empty_cache()
+
+    blocked = None
+    if not args.no_blocked:
+        blocked = blocked_section(args)
 
     line = {
-        "metric": "JW pair-comparisons/s at 1M x 1M, 3 string vars (plain linkage)",
+        "metric": "JW pair-comparisons/s at 1M x 1M, 3 string vars (plain linkage); batched-block vs plain speedup",
         "value": value,
         "unit": "comparisons/s",
         "n_gpus": 1,
@@ -251,8 +281,10 @@ def main():
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic (seeded generator, lognormal lengths, 30% perturbed copies)",
-        "config": {"workload": f"plain linkage {n} x {n} records, 3 string vars + int32 house number, "
-                               "thresholds [0.88, 0.94], p = 0.1", "n_a": n, "n_b": n, "method": args.method,
+        "config": {"workload": f"plain linkage {n} x {n} records, 3 string vars + int32 house number, thresholds "
+                               "[0.88, 0.94], p = 0.1; step = Comparison.fit from device-resident packed columns "
+                               "(device grouping of all 8 columns, all-pairs JW, expansion, sort, exact pairs, pattern "
+                               "merge)", "n_a": n, "n_b": n, "method": args.method,
                    "string_vars": list(STRING_VARS), "exact_vars": ["house_number"], "patterns": 54},
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -261,18 +293,19 @@ def main():
             "pairs_pruned_per_var": [s["pairs_pruned"] for s in st0],
             "candidates_per_var": [s["pairs_candidates"] for s in st0],
             "comparison_stage_ms_per_var": [s["kernel_ms"] for s in st0],
-            "scoring_ms_per_var": [s["eval_ms"] for s in st0],
             "alphabet": [s["alphabet"] for s in st0],
             "kernel_comparisons_per_s": comps / (kern_ms / 1e3),
             "level_list_sizes": level_sizes,
-            "union_pairs": int(n_union),
+            "union_pairs": n_union,
             "counts_head": counts[:6].tolist(),
             "em_ms": t_em * 1e3,
             "em_iterations": est.Iterations,
-            "encode_s": t_enc,
-            "encode_strings_s": t_enc_str,
-            "value_incl_encode": comps / (ms / 1e3 + t_enc),
+            "host_pack_and_copy_s": t_prep,
+            "value_incl_host_pack": comps / (ms / 1e3 + t_prep),
+            "step_from_grouped_ms": 1e3 * float(np.mean(g_times)),
+            "step_from_grouped_counts_equal": bool(np.array_equal(counts_g, counts)),
             "step_ms_all": [t * 1e3 for t in times],
+            "blocked": blocked,
         },
     }
     print(json.dumps(line), flush=True)
@@ -349,102 +382,108 @@ def bench_dedup(args):
     print(json.dumps(line), flush=True)
 
 
-def bench_blocked(args):
-    """Batched blocked linkage: all blocks in one persistent graph-captured
-    launch (autotuned) vs the per-block loops, with bit-exact per-block counts."""
+def blocked_section(args, recipe_loop=True, autotune_space=None):
+    """Batched blocked linkage (8M x 8M records in Zipf(1.2) blocks): the
+    batched form (one persistent launch per run, hipGraph replay, autotuned)
+    against the plain per-block launch loop over the same plan (one launch set
+    per block, stream-ordered, counts read back once) and against the
+    reference's recipe (one Comparison.fit per block), with per-block counts
+    compared for equality."""
     import numpy as np
     import torch
 
-    from faster import _ops, patterns
     from faster.blocking import BlockedLinkage, per_block_loop
-    from faster.comparison import exact_keys_link, fuzzy_keys_link
-    from faster.encode import group_exact_link, group_strings
     from faster.synthetic import STRING_VARS, blocked_tables
 
-    n = args.n or 8_000_000
-    log(f"generating {n} x {n} records in {args.blocks} Zipf(1.2) blocks")
+    n = args.blocked_n if getattr(args, "blocked_n", None) else 8_000_000
+    if getattr(args, "workload", "link") == "blocked" and args.n:
+        n = args.n
+    log(f"blocked: generating {n} x {n} records in {args.blocks} Zipf(1.2) blocks")
     t0 = time.time()
     dfa, dfb, ba, bb, sizes = blocked_tables(n, n_blocks=args.blocks, seed=args.seed)
     comps = float(np.dot(sizes, sizes)) * 3
-    log(f"generated in {time.time() - t0:.1f} s; sum of block products {comps / 3:.3e}; largest block {sizes.max()}")
+    log(f"blocked: generated in {time.time() - t0:.1f} s; sum of block products {comps / 3:.3e}; "
+        f"largest block {sizes.max()}")
     vf = list(STRING_VARS)
     t0 = time.time()
     bl = BlockedLinkage(dfa, dfb, ba, bb, vf, vf, ["house_number"], ["house_number"])
     t_enc = time.time() - t0
-    log(f"encoded in {t_enc:.1f} s; autotuning")
+    log(f"blocked: encoded in {t_enc:.1f} s; autotuning")
     t0 = time.time()
-    best, tune_res = bl.autotune(space=dict(tile_width=[128, 256], tiles_per_item=[2, 8], waves_per_item=[4, 8],
-                                            wgs_per_cu=[2, 3]))
+    space = autotune_space or dict(tile_width=[256], tiles_per_item=[4, 8], waves_per_item=[8], wgs_per_cu=[2, 3])
+    best, tune_res = bl.autotune(space=space)
     t_tune = time.time() - t0
-    log(f"autotune {t_tune:.1f} s: best {best}")
-    for w in range(args.warmup):
+    log(f"blocked: autotune {t_tune:.1f} s: best {best}")
+    for _ in range(2):
         bl.fit(use_graph=True)
     walls, devs = [], []
-    for k in range(args.steps):
+    for k in range(5):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         bl.fit(use_graph=True)
         torch.cuda.synchronize()
         walls.append(time.perf_counter() - t0)
         devs.append(bl.last_ms)
-        log(f"batched step {k + 1}/{args.steps}: wall {walls[-1] * 1e3:.1f} ms, device {bl.last_ms:.1f} ms")
     batched = bl.Block_Counts.copy()
     info = bl.info()
     ms = 1e3 * float(np.mean(walls))
-    extras = {"batched_wall_ms": ms, "batched_device_ms": float(np.mean(devs)), "autotune_s": t_tune,
-              "tuning": best, "tune_results": {str(k): v for k, v in tune_res.items()}, "plan": info,
-              "blocks": int(args.blocks), "largest_block": int(sizes.max()), "host_encode_s": t_enc,
-              "union_pairs": int(bl.n_union), "counts_head": bl.Counts[:6].tolist()}
-    if not args.no_loops:
-        log("per-block loop, reference recipe (one Comparison.fit per block)")
+    log(f"blocked: batched {ms:.1f} ms wall, {np.mean(devs):.1f} ms device")
+    # Plain per-block launch loop over the same plan.
+    bl.run_per_block()
+    bl.collect()
+    lw, ld = [], []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        bl.run_per_block()
+        loop_counts = bl.collect().copy()
+        torch.cuda.synchronize()
+        lw.append(time.perf_counter() - t0)
+        ld.append(bl.last_ms)
+    loop_ms = 1e3 * float(np.mean(lw))
+    loop_equal = bool(np.array_equal(loop_counts, batched))
+    log(f"blocked: per-block launch loop {loop_ms:.1f} ms wall, {np.mean(ld):.1f} ms device, counts equal: {loop_equal}")
+    achieved = info["alg_lane_ops"] / (float(np.mean(devs)) / 1e3)
+    out = {"workload": f"blocked linkage {n} x {n} records, {args.blocks} Zipf(1.2) blocks (sizes 10-60000 before "
+                       "rescaling), 3 string vars + house number",
+           "batched_wall_ms": ms, "batched_device_ms": float(np.mean(devs)),
+           "per_block_launch_loop_wall_ms": loop_ms, "per_block_launch_loop_device_ms": float(np.mean(ld)),
+           "per_block_launch_loop_counts_equal": loop_equal,
+           "speedup_vs_per_block_launch_loop": loop_ms / ms,
+           "comparisons_per_s": comps / (ms / 1e3), "largest_block": int(sizes.max()), "blocks": int(args.blocks),
+           "union_pairs": int(bl.n_union), "counts_head": bl.Counts[:6].tolist(), "tuning": best,
+           "autotune_s": t_tune, "host_encode_s": t_enc, "plan": info,
+           "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
+                        "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK,
+                        "kernel": "k_jw_blocked (persistent, all blocks)"}}
+    if recipe_loop:
+        log("blocked: the reference's recipe, one Comparison.fit per block")
         ref, t_ref = per_block_loop(dfa, dfb, ba, bb, vf, ["house_number"],
                                     progress=lambda k: log(f"  recipe loop: {k} blocks done"))
-        extras["recipe_loop_s"] = t_ref
-        extras["recipe_counts_equal"] = bool(np.array_equal(ref, batched))
-        extras["speedup_vs_recipe_loop"] = t_ref / (ms / 1e3)
-        log(f"recipe loop {t_ref:.1f} s, equal counts: {extras['recipe_counts_equal']}")
-        # Lean loop: columns pre-encoded per block; per block one launch set per
-        # variable plus the merge (syncs per launch, no host unique).
-        ia = np.arange(n)
-        starts = np.concatenate([[0], np.cumsum(sizes)])
-        enc = []
-        for b in range(args.blocks):
-            r = ia[starts[b]:starts[b + 1]]
-            g = [(group_strings(dfa[v].to_numpy()[r]).to_device(), group_strings(dfb[v].to_numpy()[r]).to_device())
-                 for v in vf]
-            ex = group_exact_link(dfa["house_number"].to_numpy()[r], dfb["house_number"].to_numpy()[r])
-            enc.append((len(r), g, ex, ex.to_device()))
-            if (b + 1) % 500 == 0:
-                log(f"  lean loop: encoded {b + 1} blocks")
-        torch.cuda.synchronize()
-        lean = np.zeros_like(batched)
-        t0 = time.perf_counter()
-        for b, (nr, g, ex, exd) in enumerate(enc):
-            keys = [fuzzy_keys_link(x, y) for x, y in g]
-            keys.append(exact_keys_link(exd[0], exd[1], exd[2], ex.n_groups, nr))
-            ids, offs, nu = _ops.pattern_merge(keys, [3, 3, 3, 2])
-            c = np.diff(np.asarray(offs))[1:]
-            lean[b, 1:] = c
-            lean[b, 0] = nr * nr - int(c.sum())
-        torch.cuda.synchronize()
-        t_lean = time.perf_counter() - t0
-        extras["lean_loop_s"] = t_lean
-        extras["lean_counts_equal"] = bool(np.array_equal(lean, batched))
-        extras["speedup_vs_lean_loop"] = t_lean / (ms / 1e3)
-        log(f"lean loop {t_lean:.1f} s, equal counts: {extras['lean_counts_equal']}")
-    achieved = info["alg_lane_ops"] / (float(np.mean(devs)) / 1e3)
+        out["recipe_loop_s"] = t_ref
+        out["recipe_counts_equal"] = bool(np.array_equal(ref, batched))
+        out["speedup_vs_recipe_loop"] = t_ref / (ms / 1e3)
+        log(f"blocked: recipe loop {t_ref:.1f} s, equal counts: {out['recipe_counts_equal']}")
+    del bl
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_blocked(args):
+    """The batched-block section alone, as its own line (value = speedup of the
+    batched form over the plain per-block launch loop)."""
+    out = blocked_section(args, recipe_loop=not args.no_loops,
+                          autotune_space=dict(tile_width=[128, 256], tiles_per_item=[2, 8], waves_per_item=[4, 8],
+                                              wgs_per_cu=[2, 3]))
     line = {
         "metric": "batched-block vs plain per-block speedup (8M x 8M, Zipf blocks, 3 string vars)",
-        "value": extras.get("speedup_vs_recipe_loop"), "unit": "x", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "value": out["speedup_vs_per_block_launch_loop"], "unit": "x", "n_gpus": 1, "steps": 5,
+        "warmup": 2, "ms_per_step": out["batched_wall_ms"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8", "data": "synthetic (Zipf(1.2) block sizes, 30% perturbed copies)",
-        "config": {"workload": f"blocked linkage {n} x {n} records, {args.blocks} blocks, 3 string vars + "
-                               "house number", "n": n, "blocks": args.blocks},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
-                     "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK,
-                     "traffic": measured_traffic("blocked", n), "kernel": "k_jw_blocked (persistent, all blocks)"},
+        "config": {"workload": out["workload"]},
+        "roofline": dict(out["roofline"], traffic=measured_traffic("blocked", args.n or 8_000_000)),
         "cpu_baseline": None,
-        "extras": dict(extras, comparisons_per_s=comps / (ms / 1e3)),
+        "extras": out,
     }
     print(json.dumps(line), flush=True)
 

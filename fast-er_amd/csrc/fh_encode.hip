@@ -119,6 +119,102 @@ __global__ void __launch_bounds__(kEncThreads) k_enc_copy(const uint8_t* __restr
 
 unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kEncThreads), 8192)); }
 
+// ------------------------- numeric value grouping --------------------------
+// Order-preserving unsigned keys: signed int64 with the sign bit flipped;
+// float64 with every NaN mapped to one positive quiet NaN and -0.0 to +0.0
+// (np.unique treats them as one value each) and the IEEE sign trick, so NaN
+// sorts last as in np.unique.
+__device__ __forceinline__ uint64_t value_key(int64_t v, int kind) {
+  uint64_t u = (uint64_t)v;
+  if (kind == FH_VALUES_F64) {
+    const uint64_t mag = u & 0x7FFFFFFFFFFFFFFFull;
+    if (mag > 0x7FF0000000000000ull) u = 0x7FF8000000000000ull;  // any NaN
+    else if (mag == 0) u = 0;                                    // -0.0
+    return (u >> 63) ? ~u : (u ^ 0x8000000000000000ull);
+  }
+  return u ^ 0x8000000000000000ull;
+}
+
+__global__ void __launch_bounds__(kEncThreads) k_val_keys(const int64_t* __restrict__ values, int64_t n, int kind,
+                                                          uint64_t* __restrict__ keys, uint64_t* __restrict__ perm,
+                                                          unsigned long long* __restrict__ minmax) {
+  unsigned long long lo = ~0ull, hi = 0ull;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = value_key(values[i], kind);
+    keys[i] = k;
+    perm[i] = (uint64_t)i;
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long a = __shfl_xor(lo, d), b = __shfl_xor(hi, d);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  if (__lane_id() == 0) {
+    atomicMin(minmax, lo);
+    atomicMax(minmax + 1, hi);
+  }
+}
+
+// heads[i] = 1 where sorted position i starts a new value; rows[i] = perm[i].
+__global__ void __launch_bounds__(kEncThreads) k_val_heads(const uint64_t* __restrict__ keys,
+                                                           const uint64_t* __restrict__ perm, int64_t n,
+                                                           uint64_t* __restrict__ heads, int64_t* __restrict__ rows) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    rows[i] = (int64_t)perm[i];
+    heads[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1ull : 0ull;
+  }
+}
+
+// group[row] and starts from the scanned heads (gid = exclusive scan).
+__global__ void __launch_bounds__(kEncThreads) k_val_emit(const int64_t* __restrict__ rows,
+                                                          const uint64_t* __restrict__ heads,
+                                                          const uint64_t* __restrict__ gid, int64_t n,
+                                                          const uint64_t* __restrict__ n_groups,
+                                                          int64_t* __restrict__ group, int64_t* __restrict__ starts) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = (int64_t)(gid[i] + heads[i]) - 1;
+    group[rows[i]] = g;
+    if (heads[i]) starts[g] = i;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) starts[*n_groups] = n;
+}
+
+// B side of a grouping of the concatenation [A; B]: flag[k] = 1 for the
+// grouped entries that are B rows.
+__global__ void __launch_bounds__(kEncThreads) k_side_flags(const int64_t* __restrict__ rows, int64_t n, int64_t n_a,
+                                                            uint64_t* __restrict__ flag) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    flag[k] = rows[k] >= n_a ? 1ull : 0ull;
+}
+
+__global__ void __launch_bounds__(kEncThreads) k_side_emit(const int64_t* __restrict__ rows, int64_t n, int64_t n_a,
+                                                           const uint64_t* __restrict__ pos,
+                                                           int64_t* __restrict__ rows_b) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = rows[k];
+    if (r >= n_a) rows_b[pos[k]] = r - n_a;
+  }
+}
+
+// starts_b[g] = B entries before group g (pos at the group's first entry);
+// with `group`, also the group of every row.
+__global__ void __launch_bounds__(kEncThreads) k_side_starts(const int64_t* __restrict__ starts,
+                                                             const int64_t* __restrict__ rows, int64_t n_groups,
+                                                             int64_t n, const uint64_t* __restrict__ pos,
+                                                             const uint64_t* __restrict__ total_b,
+                                                             int64_t* __restrict__ starts_b,
+                                                             int64_t* __restrict__ group) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_groups; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k0 = starts[g], k1 = starts[g + 1];
+    starts_b[g] = (int64_t)pos[k0];
+    if (group)
+      for (int64_t k = k0; k < k1; ++k) group[rows[k]] = g;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) starts_b[n_groups] = (int64_t)*total_b;
+}
+
 }  // namespace
 }  // namespace fh
 
@@ -193,5 +289,76 @@ extern "C" int fh_group_strings(const uint8_t* chars, const int64_t* offsets, in
   FH_HIP(hipStreamSynchronize(s));
   *n_unique = (int64_t)nu;
   *unique_bytes = (int64_t)ub;
+  return FH_OK;
+}
+
+extern "C" int fh_group_values(const int64_t* values, int64_t n, int kind, int64_t* group, int64_t* starts,
+                               int64_t* rows, int64_t* n_groups, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n >= 0 && n_groups != nullptr, "fh_group_values: bad arguments");
+  FH_REQUIRE(kind == FH_VALUES_I64 || kind == FH_VALUES_F64, "fh_group_values: unknown value kind %d", kind);
+  hipStream_t s = (hipStream_t)stream;
+  if (n == 0) {
+    const int64_t zero = 0;
+    FH_HIP(hipMemcpyAsync(starts, &zero, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    FH_HIP(hipStreamSynchronize(s));
+    *n_groups = 0;
+    return FH_OK;
+  }
+  Scratch scr(s);
+  uint64_t *keys = nullptr, *keys_tmp = nullptr, *perm = nullptr, *perm_tmp = nullptr, *small = nullptr;
+  FH_TRY(scr.alloc(&keys, (size_t)n));
+  FH_TRY(scr.alloc(&keys_tmp, (size_t)n));
+  FH_TRY(scr.alloc(&perm, (size_t)n));
+  FH_TRY(scr.alloc(&perm_tmp, (size_t)n));
+  FH_TRY(scr.alloc(&small, 4));  // [0] min key, [1] max key, [2] group count
+  FH_TRY(fill_u64(small, ~0ull, 1, s));
+  FH_TRY(fill_u64(small + 1, 0, 3, s));
+  const unsigned grid = grid_for(n);
+  hipLaunchKernelGGL(k_val_keys, dim3(grid), dim3(kEncThreads), 0, s, values, n, kind, keys, perm,
+                     reinterpret_cast<unsigned long long*>(small));
+  FH_HIP(hipGetLastError());
+  uint64_t mm[2] = {0, 0};
+  FH_HIP(hipMemcpyAsync(mm, small, sizeof(mm), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  // Keys between min and max share the bits above their highest differing bit.
+  const int bits = bits_for(mm[0] ^ mm[1]);
+  if (bits > 0) FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, n, 0, bits, s));
+  uint64_t* heads = keys_tmp;
+  uint64_t* gid = perm_tmp;
+  hipLaunchKernelGGL(k_val_heads, dim3(grid), dim3(kEncThreads), 0, s, keys, perm, n, heads, rows);
+  FH_HIP(hipGetLastError());
+  FH_TRY(exclusive_scan_u64(heads, gid, n, small + 2, s));
+  hipLaunchKernelGGL(k_val_emit, dim3(grid), dim3(kEncThreads), 0, s, rows, heads, gid, n, small + 2, group, starts);
+  FH_HIP(hipGetLastError());
+  uint64_t ng = 0;
+  FH_HIP(hipMemcpyAsync(&ng, small + 2, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  *n_groups = (int64_t)ng;
+  return FH_OK;
+}
+
+extern "C" int fh_groups_side_b(const int64_t* starts, const int64_t* rows, int64_t n_groups, int64_t n_a,
+                                int64_t n_b, int64_t* starts_b, int64_t* rows_b, int64_t* group,
+                                fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_groups >= 0 && n_a >= 0 && n_b >= 0, "fh_groups_side_b: bad sizes");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t n = n_a + n_b;
+  Scratch scr(s);
+  uint64_t *pos = nullptr, *total = nullptr;
+  FH_TRY(scr.alloc(&pos, (size_t)std::max<int64_t>(1, n)));
+  FH_TRY(scr.alloc(&total, 1));
+  FH_TRY(fill_u64(total, 0, 1, s));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_side_flags, dim3(grid_for(n)), dim3(kEncThreads), 0, s, rows, n, n_a, pos);
+    FH_HIP(hipGetLastError());
+    FH_TRY(exclusive_scan_u64(pos, pos, n, total, s));
+    hipLaunchKernelGGL(k_side_emit, dim3(grid_for(n)), dim3(kEncThreads), 0, s, rows, n, n_a, pos, rows_b);
+    FH_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_side_starts, dim3(grid_for(std::max<int64_t>(1, n_groups))), dim3(kEncThreads), 0, s, starts,
+                     rows, n_groups, n, pos, total, starts_b, group);
+  FH_HIP(hipGetLastError());
   return FH_OK;
 }

@@ -135,6 +135,14 @@ struct BlockedPlan {
   WorkItem* items_dev[2] = {nullptr, nullptr};
   int64_t n_items[2] = {0, 0};
   unsigned int* heads = nullptr;
+  // The same work items grouped by block, for the per-block launch loop:
+  // block b's items of part p are items_blk_dev[p][items_blk_lo[p][b] ..
+  // items_blk_lo[p][b + 1]); its tiles of variable v are tile_blk_lo[v][b ..
+  // b + 1]; heads_blk holds one queue head per (block, part).
+  WorkItem* items_blk_dev[2] = {nullptr, nullptr};
+  std::vector<int64_t> items_blk_lo[2];
+  std::vector<std::vector<int64_t>> tile_blk_lo;
+  unsigned int* heads_blk = nullptr;
   unsigned long long* hkeys = nullptr;
   uint32_t* hcodes = nullptr;
   uint64_t hslots = 0;
@@ -193,11 +201,49 @@ struct BlockedPlan {
                          items_dev[1], n_items[1], heads + 1);
       FH_HIP(hipGetLastError());
     }
+    return enqueue_hist();
+  }
+
+  int enqueue_hist() {
     const int64_t hb = std::min<int64_t>(ceil_div((int64_t)hslots, 256), 16384);
     hipLaunchKernelGGL(k_blocked_hist, dim3((unsigned)hb), dim3(256), 0, stream, hkeys, hcodes, hslots, (uint64_t)n_b,
                        block_a, ex, n_patterns, hist, n_union);
     FH_HIP(hipGetLastError());
     return FH_OK;
+  }
+
+  // The plain per-block loop over the same tables and kernels: block after
+  // block, one table build per variable over the block's tiles and one
+  // comparison launch per string-length part over the block's work items
+  // (grid capped at the block's item count), all on the plan's stream with
+  // no host synchronisation; one histogram pass at the end.
+  int enqueue_per_block() {
+    FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(hkeys), kEmptyKey, (int64_t)hslots, stream));
+    FH_TRY(fill_u32(hcodes, 0, (int64_t)hslots, stream));
+    FH_TRY(fill_u32(heads_blk, 0, 2 * (int64_t)n_blocks, stream));
+    FH_TRY(fill_u32(overflow, 0, 1, stream));
+    FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(hist), 0, (int64_t)(n_patterns * (uint64_t)n_blocks), stream));
+    FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(n_union), 0, 1, stream));
+    for (int32_t b = 0; b < n_blocks; ++b) {
+      for (int v = 0; v < n_vars; ++v) {
+        const int64_t t0 = tile_blk_lo[v][b], t1 = tile_blk_lo[v][b + 1];
+        FH_TRY(launch_build_table(chars_b[v], off_b[v], sorted_b[v], tiles_dev[v] + t0, t1 - t0, amap_dev[v], krows[v],
+                                  tw, table[v] + (size_t)t0 * (size_t)krows[v] * (size_t)tw, stream));
+      }
+      for (int part = 0; part < 2; ++part) {
+        const int64_t i0 = items_blk_lo[part][b], ni = items_blk_lo[part][b + 1] - i0;
+        if (ni <= 0) continue;
+        const unsigned g = (unsigned)std::min<int64_t>(grid, ni);
+        if (part == 0)
+          hipLaunchKernelGGL((k_jw_blocked<1, 32>), dim3(g), dim3(kThreads), lds_bytes, stream, args_dev,
+                             items_blk_dev[0] + i0, ni, heads_blk + 2 * (int64_t)b);
+        else
+          hipLaunchKernelGGL((k_jw_blocked<33, 64>), dim3(g), dim3(kThreads), lds_bytes, stream, args_dev,
+                             items_blk_dev[1] + i0, ni, heads_blk + 2 * (int64_t)b + 1);
+        FH_HIP(hipGetLastError());
+      }
+    }
+    return enqueue_hist();
   }
 };
 
@@ -271,6 +317,7 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
   std::vector<bool> mm_done((kLongLen + 1) * (kLongLen + 1), false);
   std::vector<WorkItem> items[2];
   std::vector<int32_t> item_keys[2];  // tile kind * 256 + string-1 length of each item
+  std::vector<int32_t> item_block[2];
   int max_krows = 1;
   pl->args.resize(n_fuzzy);
 
@@ -398,6 +445,7 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
             it.cost = lanes * tile_cost * (double)it.ntiles / (double)(t1 - t0);
             items[part].push_back(it);
             item_keys[part].push_back(kind_of_len(tiles[t].len) * 256 + waves[w].len);
+            item_block[part].push_back(b);
           }
         }
       }
@@ -450,6 +498,7 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
     pl->amap_dev.push_back(d_small);
     pl->krows.push_back(krows);
     pl->table.push_back(d_table);
+    pl->tile_blk_lo.push_back(tile_lo);
   }
   // Pruning table (shared by the variables), hash table, counters.
   for (int v = 0; v < n_fuzzy; ++v)
@@ -496,7 +545,27 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
     if (!items[part].empty())
       FH_HIP(hipMemcpyAsync(pl->items_dev[part], items[part].data(), items[part].size() * sizeof(WorkItem),
                             hipMemcpyHostToDevice, s));
+    // Grouped by block (the largest-first order kept inside a block).
+    std::vector<int32_t> blk_sorted(order.size());
+    for (size_t k = 0; k < order.size(); ++k) blk_sorted[k] = item_block[part][order[k]];
+    std::vector<size_t> by_block(order.size());
+    for (size_t k = 0; k < by_block.size(); ++k) by_block[k] = k;
+    std::stable_sort(by_block.begin(), by_block.end(),
+                     [&](size_t x, size_t y) { return blk_sorted[x] < blk_sorted[y]; });
+    std::vector<WorkItem> blk_items(by_block.size());
+    auto& lo = pl->items_blk_lo[part];
+    lo.assign((size_t)n_blocks + 1, 0);
+    for (size_t k = 0; k < by_block.size(); ++k) {
+      blk_items[k] = items[part][by_block[k]];
+      lo[(size_t)blk_sorted[by_block[k]] + 1]++;
+    }
+    for (int32_t b = 0; b < n_blocks; ++b) lo[(size_t)b + 1] += lo[(size_t)b];
+    FH_TRY(pl->alloc(&pl->items_blk_dev[part], std::max<size_t>(1, blk_items.size())));
+    if (!blk_items.empty())
+      FH_HIP(hipMemcpyAsync(pl->items_blk_dev[part], blk_items.data(), blk_items.size() * sizeof(WorkItem),
+                            hipMemcpyHostToDevice, s));
   }
+  FH_TRY(pl->alloc(&pl->heads_blk, 2 * (size_t)n_blocks));
   const size_t qbytes = (size_t)kWaves * kQCap * sizeof(uint64_t);
   pl->lds_bytes = ((size_t)max_krows * (pl->tw + 1) + 1) * 4 + qbytes;
   FH_REQUIRE(pl->lds_bytes <= 64 * 1024, "fh_blocked_plan_create: alphabet too large for tile width %d", pl->tw);
@@ -536,6 +605,22 @@ extern "C" int fh_blocked_run(void* plan, int use_graph, fh_stream_t stream) {
   } else {
     FH_TRY(pl->enqueue());
   }
+  FH_HIP(hipEventRecord(pl->ev_t1, pl->stream));
+  FH_HIP(hipEventRecord(pl->ev_out, pl->stream));
+  FH_HIP(hipStreamWaitEvent(caller, pl->ev_out, 0));
+  return FH_OK;
+}
+
+extern "C" int fh_blocked_run_per_block(void* plan, fh_stream_t stream) {
+  using namespace fh;
+  clear_error();
+  FH_REQUIRE(plan != nullptr, "fh_blocked_run_per_block: plan is null");
+  BlockedPlan* pl = static_cast<BlockedPlan*>(plan);
+  hipStream_t caller = (hipStream_t)stream;
+  FH_HIP(hipEventRecord(pl->ev_in, caller));
+  FH_HIP(hipStreamWaitEvent(pl->stream, pl->ev_in, 0));
+  FH_HIP(hipEventRecord(pl->ev_t0, pl->stream));
+  FH_TRY(pl->enqueue_per_block());
   FH_HIP(hipEventRecord(pl->ev_t1, pl->stream));
   FH_HIP(hipEventRecord(pl->ev_out, pl->stream));
   FH_HIP(hipStreamWaitEvent(caller, pl->ev_out, 0));

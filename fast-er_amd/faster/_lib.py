@@ -70,6 +70,8 @@ _SIGS = {
     "fh_sort_u64_into": (_i32, [_vp, _vp, _i64, _i32, _i32, _vp]),
     "fh_group_strings": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                 _vp]),
+    "fh_group_values": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, ctypes.POINTER(_i64), _vp]),
+    "fh_groups_side_b": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp]),
     "fh_split_levels": (_i32, [_vp, _i64, _i32, _vp, ctypes.POINTER(_i64), _vp]),
     "fh_sorted_member_mask": (_i32, [_vp, _i64, _vp, _i64, _i32, _vp, _vp]),
     "fh_pattern_merge": (_i32, [_i32, ctypes.POINTER(_vp), ctypes.POINTER(_i64),
@@ -110,6 +112,7 @@ _SIGS.update({
                                       ctypes.POINTER(_vp), ctypes.POINTER(_vp), _f32, _f32, _f32,
                                       ctypes.POINTER(BlockedTuning), ctypes.POINTER(_vp), _vp]),
     "fh_blocked_run": (_i32, [_vp, _i32, _vp]),
+    "fh_blocked_run_per_block": (_i32, [_vp, _vp]),
     "fh_blocked_counts": (_i32, [_vp, _vp, ctypes.POINTER(_i64), ctypes.POINTER(_i32), ctypes.POINTER(_f64)]),
     "fh_blocked_info": (_i32, [_vp, ctypes.POINTER(_f64)]),
     "fh_blocked_plan_destroy": (_i32, [_vp]),

@@ -111,6 +111,24 @@ def exact_link(group_a, starts_b, rows_b, n_groups, n_b, capacity=None):
     return out[:need]
 
 
+def exact_link_total(group_a, starts_b, rows_b, n_groups, n_b) -> int:
+    """Number of exact-equality pairs (fh_exact_link's count pass only)."""
+    L = _lib.load()
+    m = ctypes.c_int64(0)
+    check(L.fh_exact_link(ptr(group_a), _i64(group_a.numel()), ptr(starts_b), ptr(rows_b), _i64(n_groups), _i64(n_b),
+                          ptr(None), _i64(0), ctypes.byref(m), stream()), "fh_exact_link")
+    return m.value
+
+
+def exact_dedup_total(group, starts, rows, n_groups) -> int:
+    """Number of exact-equality pairs i > j (fh_exact_dedup's count pass only)."""
+    L = _lib.load()
+    m = ctypes.c_int64(0)
+    check(L.fh_exact_dedup(ptr(group), _i64(group.numel()), ptr(starts), ptr(rows), _i64(n_groups), ptr(None), _i64(0),
+                           ctypes.byref(m), stream()), "fh_exact_dedup")
+    return m.value
+
+
 def exact_dedup(group, starts, rows, n_groups, capacity=None):
     L = _lib.load()
     n = group.numel()
@@ -142,6 +160,39 @@ def group_strings(chars: torch.Tensor, offsets: torch.Tensor, n: int):
                              ctypes.byref(nu), ctypes.byref(ub), stream()), "fh_group_strings")
     k = nu.value
     return uchars[:max(ub.value, 1)], uoffsets[:k + 1], starts[:k + 1], rows[:n], k
+
+
+FH_VALUES_I64 = 0
+FH_VALUES_F64 = 1
+
+
+def group_values(values: torch.Tensor, kind: int):
+    """Device grouping of n numeric values (fh_group_values; ``values`` holds
+    int64 values or float64 bit patterns). Returns (group, starts, rows,
+    n_groups)."""
+    L = _lib.load()
+    n = values.numel()
+    group = _empty_i64(n)
+    starts = _empty_i64(n + 1)
+    rows = _empty_i64(n)
+    ng = ctypes.c_int64(0)
+    v = values.contiguous()
+    check(L.fh_group_values(ptr(v) if n else ptr(None), _i64(n), int(kind), ptr(group), ptr(starts), ptr(rows),
+                            ctypes.byref(ng), stream()), "fh_group_values")
+    k = ng.value
+    return group[:n], starts[:k + 1], rows[:n], k
+
+
+def groups_side_b(starts: torch.Tensor, rows: torch.Tensor, n_groups: int, n_a: int, n_b: int, want_group=False):
+    """B rows per group of a grouping of [A rows; B rows] (fh_groups_side_b).
+    Returns (starts_b, rows_b, group or None)."""
+    L = _lib.load()
+    starts_b = _empty_i64(n_groups + 1)
+    rows_b = _empty_i64(n_b)
+    group = _empty_i64(n_a + n_b) if want_group else None
+    check(L.fh_groups_side_b(ptr(starts), ptr(rows), _i64(n_groups), _i64(n_a), _i64(n_b), ptr(starts_b),
+                             ptr(rows_b), ptr(group), stream()), "fh_groups_side_b")
+    return starts_b[:n_groups + 1], rows_b[:n_b], (group[:n_a + n_b] if want_group else None)
 
 
 def sort_keys(keys: torch.Tensor, max_key: int) -> torch.Tensor:

@@ -168,6 +168,13 @@ class BlockedLinkage:
         check(_lib.load().fh_blocked_run(self.plan(), int(bool(use_graph)), stream()), "fh_blocked_run")
         return self
 
+    def run_per_block(self):
+        """Enqueues the plain per-block launch loop over the same plan (one
+        launch set per block, no graph, no host synchronisation); ``collect``
+        reads its counts. Returns self."""
+        check(_lib.load().fh_blocked_run_per_block(self.plan(), stream()), "fh_blocked_run_per_block")
+        return self
+
     def collect(self):
         """Waits for the last run; returns the per-block counts (n_blocks x
         n_patterns, int64) including the patterns without fuzzy agreement."""

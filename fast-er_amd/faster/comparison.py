@@ -12,12 +12,15 @@ so it needs neither a dense score matrix nor chunking.
 """
 from __future__ import annotations
 
+from dataclasses import dataclass
+
 import numpy as np
 import pandas as pd
 import torch
 
 from . import _lib, _ops, patterns
-from .encode import group_exact_link, group_strings_device, pack_strings
+from .encode import (exact_column, group_exact_link, group_exact_link_device, group_packed, group_strings_device,
+                     pack_strings, packed_column_device)
 from .levels import level_of_score
 
 _LEVELS_FUZZY = 3
@@ -86,10 +89,21 @@ def exact_gpu(str_A, str_B, num_threads=256):
     :return: a one-element list with the ascending int64 ids ``i * len(str_B) + j``.
     """
     _lib.load()
-    g = group_exact_link(str_A, str_B)
-    ga, sb, rb = g.to_device()
-    keys = exact_keys_link(ga, sb, rb, g.n_groups, len(np.asarray(str_B).reshape(-1)))
+    n_a = len(np.asarray(str_A).reshape(-1))
+    g = group_exact_link_device(exact_column(str_A, str_B), n_a)
+    keys = exact_keys_link(g.group_a, g.starts_b, g.rows_b, g.n_groups, len(np.asarray(str_B).reshape(-1)))
     return _split(keys, 2)
+
+
+@dataclass
+class LinkInputs:
+    """Device-resident inputs of ``Comparison.fit_device``: per fuzzy variable
+    the packed columns of A and B, per exact variable the column of A rows
+    followed by B rows."""
+    n_a: int
+    n_b: int
+    fuzzy: list
+    exact: list
 
 
 class Comparison():
@@ -142,37 +156,48 @@ class Comparison():
         if self._Fit_flag:
             raise Exception("If the model has already been fitted, it cannot be fitted again.")
         _lib.load()
-        n_a, n_b = len(self.df_A), len(self.df_B)
+        self.fit_device(self.device_inputs(), p, Lower_Thr, Upper_Thr, Exact_Mode=Exact_Mode)
+
+    def device_inputs(self):
+        """The host part of ``fit``: every compared column packed (strings as
+        UTF-8 bytes + offsets, exact columns as int64 / float64 values of A
+        then B) and copied to the device."""
+        _lib.load()
+        fuzzy = [(packed_column_device(self.df_A[va].to_numpy()), packed_column_device(self.df_B[vb].to_numpy()))
+                 for va, vb in zip(self.Vars_Fuzzy_A, self.Vars_Fuzzy_B)]
+        exact = [exact_column(self.df_A[va].to_numpy(), self.df_B[vb].to_numpy())
+                 for va, vb in zip(self.Vars_Exact_A, self.Vars_Exact_B)]
+        return LinkInputs(len(self.df_A), len(self.df_B), fuzzy, exact)
+
+    def fit_device(self, inputs, p=0.1, Lower_Thr=0.88, Upper_Thr=0.94, *, Exact_Mode="auto"):
+        """The device part of ``fit``, from ``device_inputs()``: grouping of
+        every column by value, all-pairs comparison, expansion, exact pairs and
+        the pattern merge."""
+        if self._Fit_flag:
+            raise Exception("If the model has already been fitted, it cannot be fitted again.")
+        n_a, n_b = inputs.n_a, inputs.n_b
         stats = []
         keys = []
-        for va, vb in zip(self.Vars_Fuzzy_A, self.Vars_Fuzzy_B):
-            ga = group_strings_device(self.df_A[va].to_numpy())
-            gb = group_strings_device(self.df_B[vb].to_numpy())
-            keys.append(fuzzy_keys_link(ga, gb, p, Lower_Thr, Upper_Thr, stats))
-            del ga, gb
-        exact = [group_exact_link(self.df_A[va].to_numpy(), self.df_B[vb].to_numpy())
-                 for va, vb in zip(self.Vars_Exact_A, self.Vars_Exact_B)]
-        totals = []
-        for g in exact:
-            ca = np.bincount(g.group_a, minlength=g.n_groups).astype(np.int64)
-            totals.append(int(np.dot(ca, np.diff(g.starts_b))))
-        mode = patterns.choose_exact_mode(Exact_Mode, len(self.Vars_Fuzzy_A), totals)
+        for ca, cb in inputs.fuzzy:
+            keys.append(fuzzy_keys_link(group_packed(ca), group_packed(cb), p, Lower_Thr, Upper_Thr, stats))
+        exact = [group_exact_link_device(c, n_a) for c in inputs.exact]
+        totals = [_ops.exact_link_total(g.group_a, g.starts_b, g.rows_b, g.n_groups, n_b) for g in exact]
+        mode = patterns.choose_exact_mode(Exact_Mode, len(inputs.fuzzy), totals)
         self.Exact_Mode = mode
         self.Stats = stats
-        lv_f = [_LEVELS_FUZZY] * len(self.Vars_Fuzzy_A)
+        lv_f = [_LEVELS_FUZZY] * len(inputs.fuzzy)
         self._var_levels = lv_f + [_LEVELS_EXACT] * len(exact)
         self._n_pairs = n_a * n_b
         if mode == "rowid":
-            dev = [g.to_device() for g in exact]
-            dev_b = [torch.from_numpy(g.group_b).to(_lib.device()) for g in exact]
-            rowid_dev = [(d[0], b) for d, b in zip(dev, dev_b)]
-            rowid_host = [(g.group_a, g.group_b) for g in exact]
+            rowid_dev = [(g.group_a, g.group_b) for g in exact]
+            host = [g.host() for g in exact]
+            rowid_host = [(h.group_a, h.group_b) for h in host]
 
             def equal_pairs(S):
                 if len(S) == 1:
-                    ga, sb, rb = dev[S[0]]
-                    return exact_keys_link(ga, sb, rb, exact[S[0]].n_groups, n_b)
-                ja, jb = patterns.joint_ids([exact[q].group_a for q in S], [exact[q].group_b for q in S])
+                    g = exact[S[0]]
+                    return exact_keys_link(g.group_a, g.starts_b, g.rows_b, g.n_groups, n_b)
+                ja, jb = patterns.joint_ids([host[q].group_a for q in S], [host[q].group_b for q in S])
                 g = group_exact_link(ja, jb)
                 ga, sb, rb = g.to_device()
                 return exact_keys_link(ga, sb, rb, g.n_groups, n_b)
@@ -185,14 +210,14 @@ class Comparison():
             self._pattern_ids = ids
             self._pattern_offsets = offs
             self._var_keys = keys
-            self._rowid_exact = (exact, dev, totals)
+            self._rowid_exact = (exact, totals)
             self._Fit_flag = True
         else:
-            for g in exact:
-                ga, sb, rb = g.to_device()
-                keys.append(exact_keys_link(ga, sb, rb, g.n_groups, n_b))
+            for g, t in zip(exact, totals):
+                keys.append(_ops.exact_link(g.group_a, g.starts_b, g.rows_b, g.n_groups, n_b, capacity=t))
             self._rowid_exact = None
             self._finish(keys, n_a * n_b)
+        return self
 
     def _finish(self, keys, n_pairs):
         ids, offs, n_union = _ops.pattern_merge(keys, self._var_levels)
@@ -228,11 +253,11 @@ class Comparison():
         except AttributeError:
             out = [_split(k, L) for k, L in zip(self._var_keys, self._var_levels)]
             if self._rowid_exact is not None:
-                exact, dev, totals = self._rowid_exact
+                exact, totals = self._rowid_exact
                 n_b = len(self.df_B)
-                for g, d, t in zip(exact, dev, totals):
-                    out.append([patterns.RowIdPairs(t, lambda g=g, d=d: _split(
-                        exact_keys_link(d[0], d[1], d[2], g.n_groups, n_b), 2)[0])])
+                for g, t in zip(exact, totals):
+                    out.append([patterns.RowIdPairs(t, lambda g=g: _split(
+                        exact_keys_link(g.group_a, g.starts_b, g.rows_b, g.n_groups, n_b), 2)[0])])
             self._Levels = out
             return self._Levels
 

@@ -10,13 +10,16 @@ reference's forced diagonal (faster/deduplication.py:185-190).
 """
 from __future__ import annotations
 
+from dataclasses import dataclass
+
 import numpy as np
 import pandas as pd
 import torch
 
 from . import _lib, _ops, patterns
 from .comparison import _split
-from .encode import group_exact_dedup, group_strings_device
+from .encode import (exact_column, group_exact_dedup, group_exact_dedup_device, group_packed, group_strings_device,
+                     packed_column_device)
 from .levels import level_of_score
 
 _LEVELS_FUZZY = 3
@@ -55,9 +58,16 @@ def exact_dedup_gpu(string, num_threads=256):
     :return: a one-element list with the ascending int64 ids ``i * len(string) + j``, i > j.
     """
     _lib.load()
-    g = group_exact_dedup(string)
-    grp, st, rows = g.to_device()
-    return _split(_ops.exact_dedup(grp, st, rows, g.n_groups), 2)
+    g = group_exact_dedup_device(exact_column(string))
+    return _split(_ops.exact_dedup(g.group, g.starts, g.rows, g.n_groups), 2)
+
+
+@dataclass
+class DedupInputs:
+    """Device-resident inputs of ``Deduplication.fit_device``."""
+    n: int
+    fuzzy: list
+    exact: list
 
 
 class Deduplication():
@@ -93,28 +103,38 @@ class Deduplication():
         if self._Fit_flag:
             raise Exception("If the model has already been fitted, it cannot be fitted again.")
         _lib.load()
-        n = len(self.df)
+        self.fit_device(self.device_inputs(), p, Lower_Thr, Upper_Thr, Exact_Mode=Exact_Mode)
+
+    def device_inputs(self):
+        """The host part of ``fit``: every compared column packed and copied to
+        the device."""
+        _lib.load()
+        fuzzy = [packed_column_device(self.df[v].to_numpy()) for v in self.Vars_Fuzzy]
+        exact = [exact_column(self.df[v].to_numpy()) for v in self.Vars_Exact]
+        return DedupInputs(len(self.df), fuzzy, exact)
+
+    def fit_device(self, inputs, p=0.1, Lower_Thr=0.88, Upper_Thr=0.94, *, Exact_Mode="auto"):
+        """The device part of ``fit``, from ``device_inputs()``."""
+        if self._Fit_flag:
+            raise Exception("If the model has already been fitted, it cannot be fitted again.")
+        n = inputs.n
         stats = []
-        keys = []
-        for v in self.Vars_Fuzzy:
-            g = group_strings_device(self.df[v].to_numpy())
-            keys.append(fuzzy_keys_dedup(g, p, Lower_Thr, Upper_Thr, stats))
-            del g
-        exact = [group_exact_dedup(self.df[v].to_numpy()) for v in self.Vars_Exact]
-        totals = [int(np.sum(np.diff(g.starts) * (np.diff(g.starts) - 1) // 2)) for g in exact]
-        mode = patterns.choose_exact_mode(Exact_Mode, len(self.Vars_Fuzzy), totals)
+        keys = [fuzzy_keys_dedup(group_packed(c), p, Lower_Thr, Upper_Thr, stats) for c in inputs.fuzzy]
+        exact = [group_exact_dedup_device(c) for c in inputs.exact]
+        totals = [_ops.exact_dedup_total(g.group, g.starts, g.rows, g.n_groups) for g in exact]
+        mode = patterns.choose_exact_mode(Exact_Mode, len(inputs.fuzzy), totals)
         self.Exact_Mode = mode
-        lv_f = [_LEVELS_FUZZY] * len(self.Vars_Fuzzy)
+        lv_f = [_LEVELS_FUZZY] * len(inputs.fuzzy)
         if mode == "rowid":
-            dev = [g.to_device() for g in exact]
-            rowid_dev = [(d[0], d[0]) for d in dev]
-            rowid_host = [(g.group, g.group) for g in exact]
+            rowid_dev = [(g.group, g.group) for g in exact]
+            host = [g.host() for g in exact]
+            rowid_host = [(h.group, h.group) for h in host]
 
             def equal_pairs(S):
                 if len(S) == 1:
-                    grp, st, rows = dev[S[0]]
-                    return _ops.exact_dedup(grp, st, rows, exact[S[0]].n_groups)
-                g = group_exact_dedup(patterns.joint_ids([exact[q].group for q in S]))
+                    g = exact[S[0]]
+                    return _ops.exact_dedup(g.group, g.starts, g.rows, g.n_groups)
+                g = group_exact_dedup(patterns.joint_ids([host[q].group for q in S]))
                 grp, st, rows = g.to_device()
                 return _ops.exact_dedup(grp, st, rows, g.n_groups)
 
@@ -123,20 +143,21 @@ class Deduplication():
 
             ids, offs, self.Indices, self._Counts = patterns.merge_with_rowids(
                 keys, lv_f, rowid_dev, rowid_host, n, True, n * (n + 1) // 2, mat)
-            self._rowid_exact = (exact, dev, totals)
+            self._rowid_exact = (exact, totals)
         else:
-            for g in exact:
-                grp, st, rows = g.to_device()
-                keys.append(_ops.exact_dedup(grp, st, rows, g.n_groups))
+            for g, t in zip(exact, totals):
+                keys.append(_ops.exact_dedup(g.group, g.starts, g.rows, g.n_groups, capacity=t))
             ids, offs, _ = _ops.pattern_merge(keys, lv_f + [_LEVELS_EXACT] * len(exact))
             self.Indices = [ids[offs[k]:offs[k + 1]] for k in range(1, len(offs) - 1)]
             self._rowid_exact = None
+        self._n = n
         self._pattern_ids = ids
         self._pattern_offsets = offs
         self._var_keys = keys
         self._var_levels = lv_f + [_LEVELS_EXACT] * len(exact)
         self.Stats = stats
         self._Fit_flag = True
+        return self
 
     @property
     def Counts(self):
@@ -164,9 +185,9 @@ class Deduplication():
         except AttributeError:
             out = [_split(k, L) for k, L in zip(self._var_keys, self._var_levels)]
             if self._rowid_exact is not None:
-                exact, dev, totals = self._rowid_exact
-                for g, d, t in zip(exact, dev, totals):
-                    out.append([patterns.RowIdPairs(t, lambda g=g, d=d: _split(
-                        _ops.exact_dedup(d[0], d[1], d[2], g.n_groups), 2)[0])])
+                exact, totals = self._rowid_exact
+                for g, t in zip(exact, totals):
+                    out.append([patterns.RowIdPairs(t, lambda g=g: _split(
+                        _ops.exact_dedup(g.group, g.starts, g.rows, g.n_groups), 2)[0])])
             self._Levels = out
             return self._Levels

@@ -100,18 +100,36 @@ def pack_column(values) -> tuple[np.ndarray, np.ndarray]:
     return pack_strings([s if isinstance(s, str) else str(s) for s in arr])
 
 
+@dataclass
+class PackedColumn:
+    """A string column's packed bytes and int64 offsets on the device."""
+    n: int
+    chars: torch.Tensor
+    offsets: torch.Tensor
+
+
+def packed_column_device(values) -> PackedColumn:
+    """Packs a string column on the host (Arrow buffers) and copies it to the
+    device."""
+    chars, offsets = pack_column(values)
+    dev = _lib.device()
+    return PackedColumn(len(offsets) - 1, torch.from_numpy(chars if chars.size else np.zeros(1, np.uint8)).to(dev),
+                        torch.from_numpy(offsets).to(dev))
+
+
+def group_packed(col: PackedColumn) -> DeviceStringGroups:
+    """Unique values of a device-resident packed column with their rows
+    (ascending), grouped on the device (fh_group_strings)."""
+    from . import _ops
+
+    uchars, uoff, starts, rows, nu = _ops.group_strings(col.chars, col.offsets, col.n)
+    return DeviceStringGroups(n_rows=col.n, n_unique=nu, chars=uchars, offsets=uoff, starts=starts, rows=rows)
+
+
 def group_strings_device(values) -> DeviceStringGroups:
     """Unique values of a string column with their rows (ascending), grouped
     on the device (fh_group_strings): the host only packs the bytes."""
-    from . import _ops
-
-    chars, offsets = pack_column(values)
-    n = len(offsets) - 1
-    dev = _lib.device()
-    d_chars = torch.from_numpy(chars if chars.size else np.zeros(1, np.uint8)).to(dev)
-    d_off = torch.from_numpy(offsets).to(dev)
-    uchars, uoff, starts, rows, nu = _ops.group_strings(d_chars, d_off, n)
-    return DeviceStringGroups(n_rows=n, n_unique=nu, chars=uchars, offsets=uoff, starts=starts, rows=rows)
+    return group_packed(packed_column_device(values))
 
 
 def group_strings(values) -> StringGroups:
@@ -204,3 +222,140 @@ def group_exact_dedup(values) -> ExactDedupGroups:
     np.cumsum(cnt, out=starts[1:])
     rows = np.argsort(inv, kind="stable").astype(np.int64)
     return ExactDedupGroups(inv, starts, rows, len(cnt))
+
+
+# ---------------------------------------------------------------------------
+# Exact-match grouping on the device (fh_group_values / fh_group_strings and
+# fh_groups_side_b): the role of the host np.unique calls at
+# faster/comparison.py:519,535,548 and faster/deduplication.py:645.
+# ---------------------------------------------------------------------------
+def _numeric_view(values):
+    """(int64 array, value kind) for a numeric, boolean or datetime column; None
+    for anything else (compared as strings)."""
+    from . import _ops
+
+    a = np.asarray(values).reshape(-1)
+    if a.dtype.kind == "O":
+        import pandas as pd
+
+        inferred = pd.api.types.infer_dtype(a, skipna=False)
+        if inferred in ("integer", "boolean"):
+            try:
+                return a.astype(np.int64), _ops.FH_VALUES_I64
+            except (OverflowError, TypeError, ValueError):
+                return None
+        if inferred in ("floating", "mixed-integer-float"):
+            return a.astype(np.float64).view(np.int64), _ops.FH_VALUES_F64
+        return None
+    if a.dtype.kind in "biu":
+        return a.astype(np.int64, copy=False), _ops.FH_VALUES_I64
+    if a.dtype.kind in "mM":
+        return a.view(np.int64), _ops.FH_VALUES_I64
+    if a.dtype.kind == "f":
+        return a.astype(np.float64).view(np.int64), _ops.FH_VALUES_F64
+    return None
+
+
+@dataclass
+class ExactColumn:
+    """One exact-match column (or the concatenation of two) resident on the
+    device: int64 values with their kind, or packed strings."""
+    n: int
+    values: torch.Tensor = None
+    kind: int = 0
+    chars: torch.Tensor = None
+    offsets: torch.Tensor = None
+
+
+def exact_column(values_a, values_b=None) -> ExactColumn:
+    """Uploads a column (with ``values_b``: the rows of A followed by those of
+    B) for device grouping. Numeric columns travel as int64 / float64 bits;
+    other values are compared as their ``str()``."""
+    from . import _ops
+
+    dev = _lib.device()
+    cols = [values_a] if values_b is None else [values_a, values_b]
+    views = [_numeric_view(v) for v in cols]
+    if all(v is not None for v in views):
+        if len({k for _, k in views}) == 1:
+            arr, kind = np.concatenate([a for a, _ in views]), views[0][1]
+        else:  # integers on one side, floats on the other: compared as float64
+            arr = np.concatenate([np.asarray(c).reshape(-1).astype(np.float64) for c in cols]).view(np.int64)
+            kind = _ops.FH_VALUES_F64
+        return ExactColumn(n=len(arr), values=torch.from_numpy(np.ascontiguousarray(arr)).to(dev), kind=kind)
+    packed = [pack_column(c) for c in cols]
+    if len(packed) > 1:
+        (ca, oa), (cb, ob) = packed
+        chars = np.concatenate([ca, cb])
+        offs = np.concatenate([oa, ob[1:] + oa[-1]])
+    else:
+        chars, offs = packed[0]
+    return ExactColumn(n=len(offs) - 1,
+                       chars=torch.from_numpy(chars if chars.size else np.zeros(1, np.uint8)).to(dev),
+                       offsets=torch.from_numpy(offs).to(dev))
+
+
+def _group_column(col: ExactColumn):
+    """(group per row or None, starts, rows, n_groups) of a device column."""
+    from . import _ops
+
+    if col.values is not None:
+        return _ops.group_values(col.values, col.kind)
+    _, _, starts, rows, k = _ops.group_strings(col.chars, col.offsets, col.n)
+    return None, starts, rows, k
+
+
+@dataclass
+class DeviceExactLinkGroups:
+    """Exact-match grouping of two columns over their joint values, on the
+    device: value id of every A and B row, and B rows per value (ascending)."""
+    n_groups: int
+    group_a: torch.Tensor
+    group_b: torch.Tensor
+    starts_b: torch.Tensor
+    rows_b: torch.Tensor
+
+    def to_device(self):
+        return self.group_a, self.starts_b, self.rows_b
+
+    def host(self) -> ExactLinkGroups:
+        return ExactLinkGroups(self.group_a.cpu().numpy(), self.starts_b.cpu().numpy(), self.rows_b.cpu().numpy(),
+                               self.n_groups, self.group_b.cpu().numpy())
+
+
+def group_exact_link_device(col: ExactColumn, n_a: int) -> DeviceExactLinkGroups:
+    """Joint value ids of the rows of ``col`` (A rows then B rows) and the B
+    rows of every value, all on the device."""
+    from . import _ops
+
+    n_b = col.n - n_a
+    group, starts, rows, k = _group_column(col)
+    starts_b, rows_b, g2 = _ops.groups_side_b(starts, rows, k, n_a, n_b, want_group=group is None)
+    if group is None:
+        group = g2
+    return DeviceExactLinkGroups(k, group[:n_a], group[n_a:], starts_b, rows_b)
+
+
+@dataclass
+class DeviceExactDedupGroups:
+    n_groups: int
+    group: torch.Tensor
+    starts: torch.Tensor
+    rows: torch.Tensor
+
+    def to_device(self):
+        return self.group, self.starts, self.rows
+
+    def host(self) -> ExactDedupGroups:
+        return ExactDedupGroups(self.group.cpu().numpy(), self.starts.cpu().numpy(), self.rows.cpu().numpy(),
+                                self.n_groups)
+
+
+def group_exact_dedup_device(col: ExactColumn) -> DeviceExactDedupGroups:
+    """Value id of every row and the rows of every value, on the device."""
+    from . import _ops
+
+    group, starts, rows, k = _group_column(col)
+    if group is None:
+        _, _, group = _ops.groups_side_b(starts, rows, k, col.n, 0, want_group=True)
+    return DeviceExactDedupGroups(k, group, starts, rows)

@@ -165,6 +165,16 @@ int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a, const int
                            void** plan, fh_stream_t stream);
 int fh_blocked_run(void* plan, int use_graph, fh_stream_t stream);
 int fh_blocked_counts(void* plan, int64_t* hist, int64_t* n_union, int* overflow, double* ms);
+/*
+ * The plain per-block launch loop over the same plan, for comparison with the
+ * batched launch: the reference's recipe runs one Comparison.fit per block
+ * (docs/source/usage.rst:41-76); here block after block gets one table build
+ * per fuzzy variable over its tiles and one comparison launch per string-length
+ * part over its work items, enqueued on the plan's stream with no graph and no
+ * host synchronisation. The counts accumulate on the device and are read by
+ * fh_blocked_counts exactly as after fh_blocked_run (same results).
+ */
+int fh_blocked_run_per_block(void* plan, fh_stream_t stream);
 /* info (host double[8]): work items (short, long strings), patterns, table
  * slots, pairs evaluated, algorithmic lane-ops, grid size, LDS bytes. */
 int fh_blocked_info(void* plan, double* info);
@@ -186,6 +196,36 @@ int fh_blocked_plan_destroy(void* plan);
 int fh_group_strings(const uint8_t* chars, const int64_t* offsets, int64_t n, uint8_t* uchars,
                      int64_t* uoffsets, int64_t* starts, int64_t* rows, int64_t* n_unique,
                      int64_t* unique_bytes, fh_stream_t stream);
+
+/*
+ * Groups the n rows of a numeric column by value on the device.
+ *
+ * Replaces the host np.unique(..., return_inverse=True) of an exact column
+ * (faster/comparison.py:519,535,548; faster/deduplication.py:645) with the
+ * stable argsort of its inverse. values (device, n entries) hold int64 values
+ * (kind FH_VALUES_I64; narrower integers and booleans widened by the caller)
+ * or the bit patterns of float64 values (FH_VALUES_F64; every NaN is one
+ * value and -0.0 equals +0.0, as in np.unique). Outputs (device):
+ * group[n] the value id of every row (ids in ascending value order),
+ * starts[n_groups + 1] and rows[n] the rows of value g as rows[starts[g] ..
+ * starts[g + 1]), ascending. *n_groups (host). Synchronises the stream.
+ */
+#define FH_VALUES_I64 0
+#define FH_VALUES_F64 1
+int fh_group_values(const int64_t* values, int64_t n, int kind, int64_t* group, int64_t* starts,
+                    int64_t* rows, int64_t* n_groups, fh_stream_t stream);
+
+/*
+ * The B side of a grouping of the concatenated rows [A rows; B rows] (as
+ * fh_group_values or fh_group_strings give it, rows ascending within a
+ * group): starts_b[n_groups + 1] and rows_b[n_b] list the B rows (numbered
+ * from 0) of group g as rows_b[starts_b[g] .. starts_b[g + 1]), ascending;
+ * with group non-null (n_a + n_b entries) the group of every row is written
+ * too. Replaces the joint unique and the count gathers of
+ * faster/comparison.py:548-580. Device outputs; no synchronisation.
+ */
+int fh_groups_side_b(const int64_t* starts, const int64_t* rows, int64_t n_groups, int64_t n_a, int64_t n_b,
+                     int64_t* starts_b, int64_t* rows_b, int64_t* group, fh_stream_t stream);
 
 /*
  * Unique-value pair keys -> original-row pair keys (linkage).

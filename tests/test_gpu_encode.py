@@ -80,3 +80,59 @@ def test_group_strings_1m_vs_host():
     np.testing.assert_array_equal(g.chars.cpu().numpy()[:h.chars.size], h.chars)
     np.testing.assert_array_equal(starts, h.starts)
     np.testing.assert_array_equal(rows, h.rows)
+
+
+def _np_groups(values):
+    """np.unique grouping: value id per row, rows per value (ascending)."""
+    _, inv, cnt = np.unique(values, return_inverse=True, return_counts=True)
+    inv = np.asarray(inv).reshape(-1)
+    starts = np.concatenate([[0], np.cumsum(cnt)])
+    return inv, starts, np.argsort(inv, kind="stable")
+
+
+@pytest.mark.parametrize("case", ["int32", "int64_wide", "float_nan_zero", "bool", "one", "strings", "mixed_num"])
+def test_group_exact_columns_vs_np_unique(case):
+    """Device grouping of exact columns (fh_group_values, fh_groups_side_b;
+    strings through fh_group_strings) equals np.unique's grouping of the
+    concatenated A and B values: value ids, starts and ascending rows, and the
+    B rows of every value."""
+    from faster.encode import exact_column, group_exact_dedup_device, group_exact_link_device
+
+    rng = np.random.default_rng(5)
+    n_a, n_b = 3000, 2500
+    if case == "int32":
+        a = rng.integers(-50, 50, n_a).astype(np.int32)
+        b = rng.integers(-50, 50, n_b).astype(np.int32)
+    elif case == "int64_wide":
+        pool = np.array([-(2 ** 62), -1, 0, 1, 2 ** 40, 2 ** 62 + 7], dtype=np.int64)
+        a, b = pool[rng.integers(0, 6, n_a)], pool[rng.integers(0, 6, n_b)]
+    elif case == "float_nan_zero":
+        pool = np.array([0.0, -0.0, np.nan, -np.nan, 1.5, -1.5, np.inf, -np.inf, 1e-300], dtype=np.float64)
+        a, b = pool[rng.integers(0, len(pool), n_a)], pool[rng.integers(0, len(pool), n_b)]
+    elif case == "bool":
+        a, b = rng.random(n_a) < 0.3, rng.random(n_b) < 0.6
+    elif case == "one":
+        a, b = np.full(n_a, 7), np.full(n_b, 7)
+    elif case == "strings":
+        pool = np.array(["x", "", "ab", "é", "ab\x00"], dtype=object)
+        a, b = pool[rng.integers(0, 5, n_a)], pool[rng.integers(0, 5, n_b)]
+    else:
+        a = rng.integers(0, 5, n_a).astype(np.int64)
+        b = rng.integers(0, 5, n_b).astype(np.float64) + 0.5 * (rng.random(n_b) < 0.2)
+    print(f"exact grouping {case}", flush=True)
+    both = np.concatenate([np.asarray(a, dtype=object if case == "strings" else None), b]) \
+        if case != "mixed_num" else np.concatenate([a.astype(np.float64), b])
+    inv, starts, rows = _np_groups(both)
+    g = group_exact_link_device(exact_column(a, b), n_a)
+    assert g.n_groups == len(starts) - 1
+    np.testing.assert_array_equal(g.group_a.cpu().numpy(), inv[:n_a])
+    np.testing.assert_array_equal(g.group_b.cpu().numpy(), inv[n_a:])
+    cnt_b = np.bincount(inv[n_a:], minlength=g.n_groups)
+    np.testing.assert_array_equal(g.starts_b.cpu().numpy(), np.concatenate([[0], np.cumsum(cnt_b)]))
+    np.testing.assert_array_equal(g.rows_b.cpu().numpy(), np.argsort(inv[n_a:], kind="stable"))
+    d = group_exact_dedup_device(exact_column(a))
+    inv_a, starts_a, rows_a = _np_groups(np.asarray(a, dtype=object) if case == "strings" else a)
+    assert d.n_groups == len(starts_a) - 1
+    np.testing.assert_array_equal(d.group.cpu().numpy(), inv_a)
+    np.testing.assert_array_equal(d.starts.cpu().numpy(), starts_a)
+    np.testing.assert_array_equal(d.rows.cpu().numpy(), rows_a)

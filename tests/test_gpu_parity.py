@@ -395,7 +395,8 @@ def test_rowid_exact_mode(oracle, which):
 def test_blocked_linkage_matches_per_block_loop(oracle):
     """One persistent graph-captured launch over all blocks gives the per-block
     Counts of the per-block recipe, bit for bit, on every graph replay (with
-    eager work between replays) and without the graph."""
+    eager work between replays), without the graph, and as the plain
+    per-block launch loop over the same plan."""
     from faster.blocking import BlockedLinkage, per_block_loop
     from faster.synthetic import blocked_tables
 
@@ -409,6 +410,11 @@ def test_blocked_linkage_matches_per_block_loop(oracle):
         _ = torch.arange(1000, device="cuda").sum().item()  # eager work between replays
     bl.fit(use_graph=False)
     np.testing.assert_array_equal(bl.Block_Counts, ref)
+    # The plain per-block launch loop over the same plan, twice (its queue
+    # heads and table are reset by every run).
+    for k in range(2):
+        bl.run_per_block()
+        np.testing.assert_array_equal(bl.collect(), ref, err_msg=f"per-block launch loop, run {k}")
     # The largest block against the oracle directly.
     b = int(np.argmax(sizes))
     ra, rb = np.nonzero(ba == b)[0], np.nonzero(bb == b)[0]
