@@ -137,4 +137,22 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 
 __device__ __forceinline__ int popc64(uint64_t x) { return __popcll(x); }
 
+// Order-preserving unsigned key of a value: a signed int64 with the sign bit
+// flipped; for the bit pattern of a float64, every NaN becomes one positive
+// quiet NaN and -0.0 becomes +0.0 (np.unique treats each as one value), then
+// the IEEE sign trick, so NaN sorts last as in np.unique.
+__device__ __forceinline__ uint64_t value_key(int64_t v, bool is_f64) {
+  uint64_t u = (uint64_t)v;
+  if (is_f64) {
+    const uint64_t mag = u & 0x7FFFFFFFFFFFFFFFull;
+    if (mag > 0x7FF0000000000000ull) u = 0x7FF8000000000000ull;
+    else if (mag == 0) u = 0;
+    return (u >> 63) ? ~u : (u ^ 0x8000000000000000ull);
+  }
+  return u ^ 0x8000000000000000ull;
+}
+
+// Key of the canonical NaN (the largest float64 key).
+constexpr uint64_t kNanKey = 0xFFF8000000000000ull;
+
 }  // namespace fhd

@@ -120,27 +120,12 @@ __global__ void __launch_bounds__(kEncThreads) k_enc_copy(const uint8_t* __restr
 unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kEncThreads), 8192)); }
 
 // ------------------------- numeric value grouping --------------------------
-// Order-preserving unsigned keys: signed int64 with the sign bit flipped;
-// float64 with every NaN mapped to one positive quiet NaN and -0.0 to +0.0
-// (np.unique treats them as one value each) and the IEEE sign trick, so NaN
-// sorts last as in np.unique.
-__device__ __forceinline__ uint64_t value_key(int64_t v, int kind) {
-  uint64_t u = (uint64_t)v;
-  if (kind == FH_VALUES_F64) {
-    const uint64_t mag = u & 0x7FFFFFFFFFFFFFFFull;
-    if (mag > 0x7FF0000000000000ull) u = 0x7FF8000000000000ull;  // any NaN
-    else if (mag == 0) u = 0;                                    // -0.0
-    return (u >> 63) ? ~u : (u ^ 0x8000000000000000ull);
-  }
-  return u ^ 0x8000000000000000ull;
-}
-
 __global__ void __launch_bounds__(kEncThreads) k_val_keys(const int64_t* __restrict__ values, int64_t n, int kind,
                                                           uint64_t* __restrict__ keys, uint64_t* __restrict__ perm,
                                                           unsigned long long* __restrict__ minmax) {
   unsigned long long lo = ~0ull, hi = 0ull;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t k = value_key(values[i], kind);
+    const uint64_t k = fhd::value_key(values[i], kind == FH_VALUES_F64);
     keys[i] = k;
     perm[i] = (uint64_t)i;
     lo = k < lo ? k : lo;

@@ -227,14 +227,20 @@ __global__ void __launch_bounds__(kThreads) k_exact_link(const int64_t* __restri
   });
 }
 
-// Position of every row inside its value group (rows ascending per group).
+// Position of every row inside its value group (rows ascending per group);
+// 0 for a row without a value (negative group), which pairs with nothing.
+// rows lists the starts[n_groups] rows that have a value.
 __global__ void __launch_bounds__(kThreads) k_group_rank(const int64_t* __restrict__ group,
                                                          const int64_t* __restrict__ starts,
                                                          const int64_t* __restrict__ rows, int64_t n,
-                                                         uint64_t* __restrict__ rank) {
+                                                         int64_t n_groups, uint64_t* __restrict__ rank) {
+  const int64_t listed = min(starts[n_groups], n);
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = rows[k];
-    rank[r] = (uint64_t)(k - starts[group[r]]);
+    if (group[k] < 0) rank[k] = 0;
+    if (k < listed) {
+      const int64_t r = rows[k];
+      rank[r] = (uint64_t)(k - starts[group[r]]);
+    }
   }
 }
 
@@ -645,6 +651,193 @@ __global__ void __launch_bounds__(kThreads) k_pair_posteriors(const int64_t* __r
   }
 }
 
+// ---------------------------- numeric distance ------------------------------
+// Two values x, y are within distance d when max(x, y) <= fl(min(x, y) + d)
+// in float64 (fl: round to nearest); level 2 within d2, level 1 within d1
+// (d2 <= d1) but not d2. NaN is a missing value and is compared with nothing.
+// One side is sorted by value; for a value a, its partners form one window of
+// the sorted values, found by two binary searches: below a the predicate
+// fl(b + d) >= a, above it b <= fl(a + d), both monotone in b.
+
+// Keys of the values to sort; NaNs are counted (they sort last).
+__global__ void __launch_bounds__(kThreads) k_num_keys(const double* __restrict__ v, int64_t n,
+                                                       uint64_t* __restrict__ keys, uint64_t* __restrict__ perm,
+                                                       unsigned long long* __restrict__ small) {
+  unsigned long long lo = ~0ull, hi = 0ull, nan = 0ull;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t k = fhd::value_key(__double_as_longlong(v[i]), true);
+    keys[i] = k;
+    perm[i] = (uint64_t)i;
+    lo = k < lo ? k : lo;
+    hi = k > hi ? k : hi;
+    nan += k == fhd::kNanKey ? 1ull : 0ull;
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long a = __shfl_xor(lo, d), b = __shfl_xor(hi, d), c = __shfl_xor(nan, d);
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+    nan += c;
+  }
+  if (__lane_id() == 0) {
+    atomicMin(small, lo);
+    atomicMax(small + 1, hi);
+    if (nan) atomicAdd(small + 2, nan);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_num_gather(const double* __restrict__ v, const uint64_t* __restrict__ perm,
+                                                         int64_t n, double* __restrict__ vs) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    vs[k] = v[perm[k]];
+}
+
+// First k in [lo, hi) with fl(vs[k] + d) >= a (vs ascending), else hi.
+__device__ __forceinline__ int64_t num_lower(const double* __restrict__ vs, int64_t lo, int64_t hi, double a, double d) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (__dadd_rn(vs[mid], d) >= a)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+// First k in [lo, hi) with vs[k] > t, else hi.
+__device__ __forceinline__ int64_t num_upper(const double* __restrict__ vs, int64_t lo, int64_t hi, double t) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (vs[mid] > t)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
+// Linkage: per A row its window in the sorted B values (nbv non-NaN values):
+// win[4 i .. 4 i + 3] = lo1, lo2, hi2, hi1.
+__global__ void __launch_bounds__(kThreads) k_num_link_count(const double* __restrict__ va, int64_t n_a,
+                                                             const double* __restrict__ vs, int64_t nbv, double d2,
+                                                             double d1, int64_t* __restrict__ win,
+                                                             uint64_t* __restrict__ cnt) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_a; i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = va[i];
+    if (a != a) {
+      cnt[i] = 0;
+      win[4 * i] = win[4 * i + 1] = win[4 * i + 2] = win[4 * i + 3] = 0;
+      continue;
+    }
+    const int64_t lo1 = num_lower(vs, 0, nbv, a, d1);
+    const int64_t hi1 = num_upper(vs, lo1, nbv, __dadd_rn(a, d1));
+    const int64_t lo2 = num_lower(vs, lo1, hi1, a, d2);
+    const int64_t hi2 = num_upper(vs, lo2, hi1, __dadd_rn(a, d2));
+    win[4 * i] = lo1;
+    win[4 * i + 1] = lo2;
+    win[4 * i + 2] = hi2;
+    win[4 * i + 3] = hi1;
+    cnt[i] = (uint64_t)(hi1 - lo1);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_num_link(const int64_t* __restrict__ win, int64_t n_a,
+                                                       const uint64_t* __restrict__ perm, int64_t n_b,
+                                                       const uint64_t* __restrict__ offs, int64_t total,
+                                                       uint64_t* __restrict__ out) {
+  for_each_output(offs, n_a, total, [&](int64_t o, int64_t i) {
+    const int64_t k = win[4 * i] + (o - (int64_t)offs[i]);
+    const uint64_t lvl = (k >= win[4 * i + 1] && k < win[4 * i + 2]) ? 2ull : 1ull;
+    out[o] = ((uint64_t)i * (uint64_t)n_b + perm[k]) * 4ull + lvl;
+  });
+}
+
+// Deduplication: for sorted position p (non-NaN), the partners q > p with
+// vs[q] <= fl(vs[p] + d): win[2 p] = hi2, win[2 p + 1] = hi1.
+__global__ void __launch_bounds__(kThreads) k_num_dedup_count(const double* __restrict__ vs, int64_t nv, double d2,
+                                                              double d1, int64_t* __restrict__ win,
+                                                              uint64_t* __restrict__ cnt) {
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nv; p += (int64_t)gridDim.x * blockDim.x) {
+    const double x = vs[p];
+    const int64_t hi1 = num_upper(vs, p + 1, nv, __dadd_rn(x, d1));
+    const int64_t hi2 = num_upper(vs, p + 1, hi1, __dadd_rn(x, d2));
+    win[2 * p] = hi2;
+    win[2 * p + 1] = hi1;
+    cnt[p] = (uint64_t)(hi1 - p - 1);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_num_dedup(const int64_t* __restrict__ win, int64_t nv,
+                                                        const uint64_t* __restrict__ perm, int64_t n,
+                                                        const uint64_t* __restrict__ offs, int64_t total,
+                                                        uint64_t* __restrict__ out) {
+  for_each_output(offs, nv, total, [&](int64_t o, int64_t p) {
+    const int64_t q = p + 1 + (o - (int64_t)offs[p]);
+    const uint64_t lvl = q < win[2 * p] ? 2ull : 1ull;
+    const uint64_t r1 = perm[p], r2 = perm[q];
+    const uint64_t hi = r1 > r2 ? r1 : r2, lo = r1 > r2 ? r2 : r1;
+    out[o] = (hi * (uint64_t)n + lo) * 4ull + lvl;
+  });
+}
+
+// ----------------------------- missing levels -------------------------------
+// Re-codes the pairs of a pattern-major list with the missing digits of their
+// rows: code += sum of add[v] over the variables v missing on either row, and
+// packs (code, id) into one key so that one radix sort restores pattern-major
+// order with ids ascending inside each pattern.
+constexpr int kMaxMissVars = 32;
+struct MissingAdd {
+  uint32_t add[kMaxMissVars];
+  int n_vars;
+};
+
+__global__ void __launch_bounds__(kThreads) k_missing_recode(const int64_t* __restrict__ ids, int64_t n,
+                                                             const int64_t* __restrict__ offs, int n_patterns,
+                                                             const uint32_t* __restrict__ miss_a,
+                                                             const uint32_t* __restrict__ miss_b, uint64_t id_div,
+                                                             MissingAdd ma, int id_bits, uint64_t* __restrict__ keys) {
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = n_patterns;  // offs[lo] <= o < offs[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (offs[mid] <= o)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint64_t id = (uint64_t)ids[o];
+    const uint32_t m = miss_a[id / id_div] | miss_b[id % id_div];
+    uint64_t code = (uint64_t)lo;
+    for (int v = 0; v < ma.n_vars; ++v)
+      if ((m >> v) & 1u) code += ma.add[v];
+    keys[o] = (code << id_bits) | id;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_missing_split(const uint64_t* __restrict__ keys, int64_t n, int id_bits,
+                                                            int64_t* __restrict__ ids) {
+  const uint64_t mask = (1ull << id_bits) - 1ull;
+  for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (int64_t)gridDim.x * blockDim.x)
+    ids[o] = (int64_t)(keys[o] & mask);
+}
+
+// offsets[c] = first sorted key with code >= c, c in [0, n_patterns].
+__global__ void __launch_bounds__(kThreads) k_missing_offsets(const uint64_t* __restrict__ keys, int64_t n,
+                                                              int n_patterns, int id_bits,
+                                                              int64_t* __restrict__ offsets) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > n_patterns) return;
+  const uint64_t x = (uint64_t)c << id_bits;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (keys[mid] < x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  offsets[c] = lo;
+}
+
 // Scans `cnt` (n entries) into offsets and returns the total on the host.
 int scan_total(uint64_t* cnt, int64_t n, uint64_t* total_dev, uint64_t* total, hipStream_t s) {
   FH_TRY(exclusive_scan_u64(cnt, cnt, n, total_dev, s));
@@ -753,7 +946,7 @@ extern "C" int fh_exact_dedup(const int64_t* group, int64_t n, const int64_t* st
   uint64_t *cnt = nullptr, *tot = nullptr;
   FH_TRY(scr.alloc(&cnt, (size_t)n));
   FH_TRY(scr.alloc(&tot, 1));
-  hipLaunchKernelGGL(k_group_rank, dim3(grid_for(n)), dim3(kThreads), 0, s, group, starts, rows, n, cnt);
+  hipLaunchKernelGGL(k_group_rank, dim3(grid_for(n)), dim3(kThreads), 0, s, group, starts, rows, n, n_groups, cnt);
   FH_HIP(hipGetLastError());
   uint64_t total = 0;
   FH_TRY(scan_total(cnt, n, tot, &total, s));
@@ -1001,5 +1194,152 @@ extern "C" int fh_pair_posteriors(const int64_t* pattern_offsets, int n_patterns
   hipLaunchKernelGGL(k_pair_posteriors, dim3(grid_for(n_pairs)), dim3(kThreads), 0, s, pattern_offsets, n_patterns,
                      ksi, post, n_pairs);
   FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+// Sorts n float64 values (NaNs last) into vs with their positions in perm;
+// returns the number of non-NaN values. Synchronises once.
+static int sort_values(const double* v, int64_t n, Scratch& scr, double** vs, uint64_t** perm, int64_t* n_valid,
+                       hipStream_t s) {
+  uint64_t *keys = nullptr, *keys_tmp = nullptr, *pm = nullptr, *pm_tmp = nullptr, *small = nullptr;
+  double* sorted = nullptr;
+  FH_TRY(scr.alloc(&keys, (size_t)n));
+  FH_TRY(scr.alloc(&keys_tmp, (size_t)n));
+  FH_TRY(scr.alloc(&pm, (size_t)n));
+  FH_TRY(scr.alloc(&pm_tmp, (size_t)n));
+  FH_TRY(scr.alloc(&sorted, (size_t)n));
+  FH_TRY(scr.alloc(&small, 4));
+  FH_TRY(fill_u64(small, ~0ull, 1, s));
+  FH_TRY(fill_u64(small + 1, 0, 3, s));
+  hipLaunchKernelGGL(k_num_keys, dim3(grid_for(n)), dim3(kThreads), 0, s, v, n, keys, pm,
+                     reinterpret_cast<unsigned long long*>(small));
+  FH_HIP(hipGetLastError());
+  uint64_t h[3] = {0, 0, 0};
+  FH_HIP(hipMemcpyAsync(h, small, sizeof(h), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  const int bits = bits_for(h[0] ^ h[1]);
+  if (bits > 0) FH_TRY(radix_sort_u64(keys, keys_tmp, pm, pm_tmp, n, 0, bits, s));
+  const int64_t nv = n - (int64_t)h[2];
+  if (nv > 0) {
+    hipLaunchKernelGGL(k_num_gather, dim3(grid_for(nv)), dim3(kThreads), 0, s, v, pm, nv, sorted);
+    FH_HIP(hipGetLastError());
+  }
+  *vs = sorted;
+  *perm = pm;
+  *n_valid = nv;
+  return FH_OK;
+}
+
+extern "C" int fh_numeric_link(const double* values_a, int64_t n_a, const double* values_b, int64_t n_b, double d2,
+                               double d1, uint64_t* out, int64_t capacity, int64_t* n_out, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_out != nullptr, "fh_numeric_link: n_out is null");
+  FH_REQUIRE(n_a >= 0 && n_b >= 0, "fh_numeric_link: bad sizes");
+  FH_REQUIRE(d2 >= 0.0 && d1 >= d2, "fh_numeric_link: need 0 <= d2 <= d1");
+  *n_out = 0;
+  if (n_a == 0 || n_b == 0) return FH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Scratch scr(s);
+  double* vs = nullptr;
+  uint64_t* perm = nullptr;
+  int64_t nbv = 0;
+  FH_TRY(sort_values(values_b, n_b, scr, &vs, &perm, &nbv, s));
+  uint64_t *cnt = nullptr, *tot = nullptr;
+  int64_t* win = nullptr;
+  FH_TRY(scr.alloc(&cnt, (size_t)n_a));
+  FH_TRY(scr.alloc(&tot, 1));
+  FH_TRY(scr.alloc(&win, 4 * (size_t)n_a));
+  hipLaunchKernelGGL(k_num_link_count, dim3(grid_for(n_a)), dim3(kThreads), 0, s, values_a, n_a, (const double*)vs, nbv,
+                     d2, d1, win, cnt);
+  FH_HIP(hipGetLastError());
+  uint64_t total = 0;
+  FH_TRY(scan_total(cnt, n_a, tot, &total, s));
+  *n_out = (int64_t)total;
+  if ((int64_t)total > capacity || total == 0) return FH_OK;
+  hipLaunchKernelGGL(k_num_link, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, (const int64_t*)win, n_a,
+                     (const uint64_t*)perm, n_b, cnt, (int64_t)total, out);
+  FH_HIP(hipGetLastError());
+  FH_HIP(hipStreamSynchronize(s));
+  return FH_OK;
+}
+
+extern "C" int fh_numeric_dedup(const double* values, int64_t n, double d2, double d1, uint64_t* out,
+                                int64_t capacity, int64_t* n_out, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_out != nullptr, "fh_numeric_dedup: n_out is null");
+  FH_REQUIRE(n >= 0, "fh_numeric_dedup: bad size");
+  FH_REQUIRE(d2 >= 0.0 && d1 >= d2, "fh_numeric_dedup: need 0 <= d2 <= d1");
+  *n_out = 0;
+  if (n < 2) return FH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Scratch scr(s);
+  double* vs = nullptr;
+  uint64_t* perm = nullptr;
+  int64_t nv = 0;
+  FH_TRY(sort_values(values, n, scr, &vs, &perm, &nv, s));
+  if (nv < 2) return FH_OK;
+  uint64_t *cnt = nullptr, *tot = nullptr;
+  int64_t* win = nullptr;
+  FH_TRY(scr.alloc(&cnt, (size_t)nv));
+  FH_TRY(scr.alloc(&tot, 1));
+  FH_TRY(scr.alloc(&win, 2 * (size_t)nv));
+  hipLaunchKernelGGL(k_num_dedup_count, dim3(grid_for(nv)), dim3(kThreads), 0, s, (const double*)vs, nv, d2, d1, win,
+                     cnt);
+  FH_HIP(hipGetLastError());
+  uint64_t total = 0;
+  FH_TRY(scan_total(cnt, nv, tot, &total, s));
+  *n_out = (int64_t)total;
+  if ((int64_t)total > capacity || total == 0) return FH_OK;
+  hipLaunchKernelGGL(k_num_dedup, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, (const int64_t*)win, nv,
+                     (const uint64_t*)perm, n, cnt, (int64_t)total, out);
+  FH_HIP(hipGetLastError());
+  FH_HIP(hipStreamSynchronize(s));
+  return FH_OK;
+}
+
+extern "C" int fh_pattern_add_missing(const int64_t* ids, const int64_t* pattern_offsets, int n_patterns, int64_t n,
+                                      const uint32_t* miss_a, const uint32_t* miss_b, int64_t id_div, int64_t max_id,
+                                      int n_vars, const int64_t* add, int64_t* ids_out, int64_t* offsets_out,
+                                      fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_patterns >= 1 && n >= 0 && id_div > 0 && max_id >= 0, "fh_pattern_add_missing: bad sizes");
+  FH_REQUIRE(n_vars >= 1 && n_vars <= kMaxMissVars, "fh_pattern_add_missing: 1..%d variables", kMaxMissVars);
+  FH_REQUIRE(offsets_out != nullptr && add != nullptr, "fh_pattern_add_missing: null host array");
+  MissingAdd ma;
+  ma.n_vars = n_vars;
+  int64_t max_add = 0;
+  for (int v = 0; v < n_vars; ++v) {
+    FH_REQUIRE(add[v] >= 0 && add[v] < n_patterns, "fh_pattern_add_missing: digit weight out of range");
+    ma.add[v] = (uint32_t)add[v];
+    max_add += add[v];
+  }
+  FH_REQUIRE(max_add < n_patterns, "fh_pattern_add_missing: digit weights exceed the pattern count");
+  const int id_bits = std::max(1, bits_for((uint64_t)max_id));
+  const int code_bits = bits_for((uint64_t)n_patterns);
+  FH_REQUIRE(id_bits + code_bits <= 64, "fh_pattern_add_missing: ids of %d bits and %d patterns do not fit 64 bits",
+             id_bits, n_patterns);
+  hipStream_t s = (hipStream_t)stream;
+  if (n == 0) {
+    for (int c = 0; c <= n_patterns; ++c) offsets_out[c] = 0;
+    return FH_OK;
+  }
+  Scratch scr(s);
+  uint64_t *keys = nullptr, *keys_tmp = nullptr;
+  int64_t* d_offs = nullptr;
+  FH_TRY(scr.alloc(&keys, (size_t)n));
+  FH_TRY(scr.alloc(&keys_tmp, (size_t)n));
+  FH_TRY(scr.alloc(&d_offs, (size_t)n_patterns + 1));
+  hipLaunchKernelGGL(k_missing_recode, dim3(grid_for(n)), dim3(kThreads), 0, s, ids, n, pattern_offsets, n_patterns,
+                     miss_a, miss_b, (uint64_t)id_div, ma, id_bits, keys);
+  FH_HIP(hipGetLastError());
+  FH_TRY(radix_sort_u64(keys, keys_tmp, nullptr, nullptr, n, 0, id_bits + code_bits, s));
+  hipLaunchKernelGGL(k_missing_split, dim3(grid_for(n)), dim3(kThreads), 0, s, (const uint64_t*)keys, n, id_bits,
+                     ids_out);
+  FH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_missing_offsets, dim3((unsigned)ceil_div((int64_t)n_patterns + 1, kThreads)), dim3(kThreads), 0,
+                     s, (const uint64_t*)keys, n, n_patterns, id_bits, d_offs);
+  FH_HIP(hipGetLastError());
+  FH_HIP(hipMemcpyAsync(offsets_out, d_offs, ((size_t)n_patterns + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
   return FH_OK;
 }

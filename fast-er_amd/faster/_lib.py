@@ -72,6 +72,10 @@ _SIGS = {
                                 _vp]),
     "fh_group_values": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, ctypes.POINTER(_i64), _vp]),
     "fh_groups_side_b": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp]),
+    "fh_numeric_link": (_i32, [_vp, _i64, _vp, _i64, _f64, _f64, _vp, _i64, ctypes.POINTER(_i64), _vp]),
+    "fh_numeric_dedup": (_i32, [_vp, _i64, _f64, _f64, _vp, _i64, ctypes.POINTER(_i64), _vp]),
+    "fh_pattern_add_missing": (_i32, [_vp, _vp, _i32, _i64, _vp, _vp, _i64, _i64, _i32, ctypes.POINTER(_i64), _vp,
+                                      ctypes.POINTER(_i64), _vp]),
     "fh_split_levels": (_i32, [_vp, _i64, _i32, _vp, ctypes.POINTER(_i64), _vp]),
     "fh_sorted_member_mask": (_i32, [_vp, _i64, _vp, _i64, _i32, _vp, _vp]),
     "fh_pattern_merge": (_i32, [_i32, ctypes.POINTER(_vp), ctypes.POINTER(_i64),

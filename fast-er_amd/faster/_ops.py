@@ -195,6 +195,61 @@ def groups_side_b(starts: torch.Tensor, rows: torch.Tensor, n_groups: int, n_a: 
     return starts_b[:n_groups + 1], rows_b[:n_b], (group[:n_a + n_b] if want_group else None)
 
 
+def numeric_link(values_a: torch.Tensor, values_b: torch.Tensor, d2: float, d1: float) -> torch.Tensor:
+    """Sorted level keys ((i * n_b + j) * 4 + level) of a numeric-distance
+    variable (fh_numeric_link; float64 values, NaN = missing)."""
+    L = _lib.load()
+    va = values_a.to(torch.float64).contiguous()
+    vb = values_b.to(torch.float64).contiguous()
+    n_a, n_b = va.numel(), vb.numel()
+
+    def run(cap):
+        out = _empty_i64(cap)
+        m = ctypes.c_int64(0)
+        check(L.fh_numeric_link(ptr(va) if n_a else ptr(None), _i64(n_a), ptr(vb) if n_b else ptr(None), _i64(n_b),
+                                ctypes.c_double(d2), ctypes.c_double(d1), ptr(out), _i64(cap), ctypes.byref(m),
+                                stream()), "fh_numeric_link")
+        return m.value, out
+
+    need, out = _retry(run, 0)
+    return sort_keys(out[:need], max_pair_key(n_a * n_b))
+
+
+def numeric_dedup(values: torch.Tensor, d2: float, d1: float) -> torch.Tensor:
+    """Sorted level keys ((row * n + col) * 4 + level, row > col) of a
+    numeric-distance variable within one column (fh_numeric_dedup)."""
+    L = _lib.load()
+    v = values.to(torch.float64).contiguous()
+    n = v.numel()
+
+    def run(cap):
+        out = _empty_i64(cap)
+        m = ctypes.c_int64(0)
+        check(L.fh_numeric_dedup(ptr(v) if n else ptr(None), _i64(n), ctypes.c_double(d2), ctypes.c_double(d1),
+                                 ptr(out), _i64(cap), ctypes.byref(m), stream()), "fh_numeric_dedup")
+        return m.value, out
+
+    need, out = _retry(run, 0)
+    return sort_keys(out[:need], max_pair_key(n * n))
+
+
+def pattern_add_missing(ids: torch.Tensor, offsets, miss_a: torch.Tensor, miss_b: torch.Tensor, id_div: int,
+                        max_id: int, add):
+    """Adds missing-level digits to a pattern-major pair list
+    (fh_pattern_add_missing). Returns (ids, host offsets)."""
+    L = _lib.load()
+    n_patterns = len(offsets) - 1
+    n = ids.numel()
+    d_offs = torch.tensor(offsets, dtype=torch.int64, device=_lib.device())
+    out = _empty_i64(n)
+    offs = (ctypes.c_int64 * (n_patterns + 1))()
+    addv = (ctypes.c_int64 * len(add))(*[int(x) for x in add])
+    check(L.fh_pattern_add_missing(ptr(ids) if n else ptr(None), ptr(d_offs), int(n_patterns), _i64(n), ptr(miss_a),
+                                   ptr(miss_b), _i64(id_div), _i64(max_id), len(add), addv, ptr(out), offs, stream()),
+          "fh_pattern_add_missing")
+    return out[:n], [offs[i] for i in range(n_patterns + 1)]
+
+
 def sort_keys(keys: torch.Tensor, max_key: int) -> torch.Tensor:
     """Ascending radix sort of level keys below ``max_key``: returns the
     sorted keys in a new tensor and leaves ``keys`` overwritten."""

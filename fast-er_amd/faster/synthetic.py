@@ -62,6 +62,44 @@ def perturb_all(rng, strings, min_edits, max_edits):
     return out
 
 
+def add_numeric_and_missing(dfa, dfb, truth, seed=0, missing=0.05, days=27759, p_same=0.8, max_shift=3):
+    """Extends linkage tables with a numeric variable and missing values.
+
+    ``birth_day`` is a day number uniform over ``days`` days (1930-01-01 to
+    2005-12-31); a copied B row keeps its A row's day with probability
+    ``p_same`` and is otherwise shifted by 1..``max_shift`` days either way.
+    Then every value of every compared column (the string variables,
+    ``house_number`` and ``birth_day``) is missing with probability
+    ``missing``, independently (strings become None, numbers NaN, so
+    ``house_number`` becomes float64). Draws come from their own generator, so
+    the other columns stay those of ``linkage_tables``."""
+    rng = np.random.default_rng(seed + 7919)
+    n_a, n_b = len(dfa), len(dfb)
+    a_day = rng.integers(0, days, n_a).astype(np.float64)
+    b_day = rng.integers(0, days, n_b).astype(np.float64)
+    cp = np.nonzero(truth >= 0)[0]
+    shift = rng.integers(1, max_shift + 1, len(cp)) * rng.choice([-1, 1], len(cp))
+    keep = rng.random(len(cp)) < p_same
+    b_day[cp] = a_day[truth[cp]] + np.where(keep, 0, shift)
+    dfa = dfa.copy()
+    dfb = dfb.copy()
+    dfa["birth_day"] = a_day
+    dfb["birth_day"] = b_day
+    for df in (dfa, dfb):
+        df["house_number"] = df["house_number"].astype(np.float64)
+        for v in list(STRING_VARS) + ["house_number", "birth_day"]:
+            miss = rng.random(len(df)) < missing
+            if v in STRING_VARS:
+                col = df[v].to_numpy(dtype=object).copy()
+                col[miss] = None
+                df[v] = col
+            else:
+                col = df[v].to_numpy().copy()
+                col[miss] = np.nan
+                df[v] = col
+    return dfa, dfb
+
+
 def linkage_tables(n_a, n_b, seed=0, dup_frac=0.3, max_edits=2):
     """(df_A, df_B) for linkage. Returns also the A row of every copied B row
     (-1 for independent rows)."""

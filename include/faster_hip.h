@@ -272,13 +272,51 @@ int fh_exact_link(const int64_t* group_a, int64_t n_a, const int64_t* starts_b,
  * Exact-equality pairs (deduplication), ascending.
  *
  * Replaces faster/deduplication.py:628-701 (output_count/repeat/triangular
- * decode kernel :345-391 and the sort). group[i] is the value id of row i;
- * rows of value g are rows[starts[g] .. starts[g + 1]) ascending. Output keys
+ * decode kernel :345-391 and the sort). group[i] is the value id of row i,
+ * or negative for a row without a value (it pairs with nothing); rows of
+ * value g are rows[starts[g] .. starts[g + 1]) ascending (starts[n_groups]
+ * rows in all). Output keys
  * (i * n + j) * 4 + 1 for every j < i with an equal value, ascending.
  */
 int fh_exact_dedup(const int64_t* group, int64_t n, const int64_t* starts, const int64_t* rows,
                    int64_t n_groups, uint64_t* out, int64_t capacity, int64_t* n_out,
                    fh_stream_t stream);
+
+/*
+ * Numeric-distance levels. No counterpart in the reference, which compares
+ * only by Jaro-Winkler or equality (faster/comparison.py:346-600); these give
+ * the numeric comparisons of a pattern with three levels. Values are float64
+ * (device); NaN is a missing value and pairs with nothing. Two values x, y
+ * are within distance d when max(x, y) <= fl(min(x, y) + d) (float64, round
+ * to nearest): level 2 within d2, level 1 within d1 but not d2
+ * (0 <= d2 <= d1). Output keys (id * 4 + level), unordered, into an arena as
+ * above; *n_out (host) = total. Synchronises the stream.
+ *   fh_numeric_link : every pair (i, j) of A x B, id i * n_b + j.
+ *   fh_numeric_dedup: every pair i != j of one column, id
+ *                     max(i, j) * n + min(i, j).
+ */
+int fh_numeric_link(const double* values_a, int64_t n_a, const double* values_b, int64_t n_b, double d2,
+                    double d1, uint64_t* out, int64_t capacity, int64_t* n_out, fh_stream_t stream);
+int fh_numeric_dedup(const double* values, int64_t n, double d2, double d1, uint64_t* out, int64_t capacity,
+                     int64_t* n_out, fh_stream_t stream);
+
+/*
+ * Missing levels of a pattern-major pair list. No counterpart in the
+ * reference (it has no missing values; faster/estimation.py:70 fixes the
+ * levels). ids (device int64[n]) and pattern_offsets (device
+ * int64[n_patterns + 1]) are fh_pattern_merge's output over variables whose
+ * last level is "missing" (never listed); miss_a / miss_b (device uint32) hold
+ * per row a bit per variable, set when the row's value is missing. A pair's
+ * code gets add[v] (host int64[n_vars]: the missing level times the
+ * variable's pattern stride) for every variable missing on either of its rows
+ * (rows id / id_div and id % id_div; ids <= max_id). ids_out (device
+ * int64[n]) receives the pairs pattern-major again, each pattern ascending;
+ * offsets_out (host int64[n_patterns + 1]) its boundaries. Synchronises.
+ */
+int fh_pattern_add_missing(const int64_t* ids, const int64_t* pattern_offsets, int n_patterns, int64_t n,
+                           const uint32_t* miss_a, const uint32_t* miss_b, int64_t id_div, int64_t max_id,
+                           int n_vars, const int64_t* add, int64_t* ids_out, int64_t* offsets_out,
+                           fh_stream_t stream);
 
 /*
  * Ascending sort of uint64 keys on bits [begin_bit, end_bit).

@@ -502,3 +502,176 @@ def linkage_pairs(indices, ksi, n_B, threshold=0.85):
         raise Exception("No pair of observations has a conditional match probability exceeding the threshold.")
     ids = np.concatenate([np.asarray(indices[i - 1], dtype=np.int64) for i in above])
     return ids // n_B, ids % n_B
+
+
+# ---------------------------------------------------------------------------
+# Numeric-distance and missing levels (no counterpart in the reference: its
+# comparisons are Jaro-Winkler and equality only, faster/comparison.py:346-600,
+# with the levels fixed at faster/estimation.py:70,116). Brute force over
+# every record pair, on dense level matrices; small inputs only.
+# ---------------------------------------------------------------------------
+
+def within(x, y, d):
+    """max(x, y) <= min(x, y) + d in float64 (NaN: False)."""
+    x = np.asarray(x, dtype=np.float64)
+    y = np.asarray(y, dtype=np.float64)
+    lo, hi = np.minimum(x, y), np.maximum(x, y)
+    with np.errstate(invalid="ignore"):
+        return hi <= lo + np.float64(d)
+
+
+def numeric_level_matrix(xa, xb, d2, d1):
+    """Level of every pair (i, j): 2 within d2, 1 within d1, else 0."""
+    a = np.asarray(xa, dtype=np.float64)[:, None]
+    b = np.asarray(xb, dtype=np.float64)[None, :]
+    return np.where(within(a, b, d2), 2, np.where(within(a, b, d1), 1, 0)).astype(np.int8)
+
+
+def numeric_link(xa, xb, d2, d1):
+    """[level-1 ids, level-2 ids], ids i * n_B + j ascending."""
+    M = numeric_level_matrix(xa, xb, d2, d1).ravel()
+    return [np.nonzero(M == 1)[0].astype(np.int64), np.nonzero(M == 2)[0].astype(np.int64)]
+
+
+def numeric_dedup(x, d2, d1):
+    """[level-1 ids, level-2 ids], ids i * n + j with i > j, ascending."""
+    n = len(x)
+    M = numeric_level_matrix(x, x, d2, d1)
+    M[np.triu_indices(n)] = 0
+    M = M.ravel()
+    return [np.nonzero(M == 1)[0].astype(np.int64), np.nonzero(M == 2)[0].astype(np.int64)]
+
+
+def _isna(values):
+    import pandas as pd
+
+    return np.asarray(pd.isna(pd.Series(np.asarray(values, dtype=object).reshape(-1))), dtype=bool)
+
+
+def _as_float(values):
+    import pandas as pd
+
+    return pd.to_numeric(pd.Series(np.asarray(values, dtype=object).reshape(-1))).to_numpy(np.float64, na_value=np.nan)
+
+
+def _fill(D, ids, n_cols, rows_a, rows_b, level):
+    """D[rows_a[i], rows_b[j]] = level for subset pair ids i * n_cols + j."""
+    ids = np.asarray(ids, dtype=np.int64)
+    D[rows_a[ids // n_cols], rows_b[ids % n_cols]] = level
+
+
+def _digits_link(cols_fuzzy_A, cols_fuzzy_B, cols_exact_A, cols_exact_B, cols_num_A, cols_num_B, d2, d1, missing,
+                 p, lower_thr, upper_thr):
+    n_A = len((list(cols_fuzzy_A) + list(cols_exact_A) + list(cols_num_A))[0])
+    n_B = len((list(cols_fuzzy_B) + list(cols_exact_B) + list(cols_num_B))[0])
+    digits, base = [], []
+    for a, b in zip(cols_fuzzy_A, cols_fuzzy_B):
+        a, b = np.asarray(a, dtype=object), np.asarray(b, dtype=object)
+        ma = _isna(a) if missing else np.zeros(n_A, bool)
+        mb = _isna(b) if missing else np.zeros(n_B, bool)
+        ra, rb = np.nonzero(~ma)[0], np.nonzero(~mb)[0]
+        D = np.zeros((n_A, n_B), np.int8)
+        if len(ra) and len(rb):
+            l1, l2 = jaro_winkler_all_pairs(list(a[ra]), list(b[rb]), p=p, lower_thr=lower_thr, upper_thr=upper_thr)
+            _fill(D, l1, len(rb), ra, rb, 1)
+            _fill(D, l2, len(rb), ra, rb, 2)
+        digits.append((D, ma, mb))
+        base.append(3)
+    for a, b in zip(cols_exact_A, cols_exact_B):
+        a, b = np.asarray(a, dtype=object), np.asarray(b, dtype=object)
+        ma = _isna(a) if missing else np.zeros(n_A, bool)
+        mb = _isna(b) if missing else np.zeros(n_B, bool)
+        ids = np.full(n_A + n_B, -1, dtype=np.int64)
+        keep = np.concatenate([~ma, ~mb])
+        both = np.concatenate([a, b])[keep]
+        if len(both):
+            _, inv = np.unique(both, return_inverse=True)
+            ids[keep] = np.asarray(inv).reshape(-1)
+        ia, ib = ids[:n_A], ids[n_A:]
+        D = ((ia[:, None] == ib[None, :]) & (ia[:, None] >= 0)).astype(np.int8)
+        digits.append((D, ma, mb))
+        base.append(2)
+    for a, b in zip(cols_num_A, cols_num_B):
+        xa, xb = _as_float(a), _as_float(b)
+        D = numeric_level_matrix(xa, xb, d2, d1)
+        ma = np.isnan(xa) if missing else np.zeros(n_A, bool)
+        mb = np.isnan(xb) if missing else np.zeros(n_B, bool)
+        digits.append((D, ma, mb))
+        base.append(3)
+    return n_A, n_B, digits, base
+
+
+def _patterns_from_digits(digits, base, missing, pair_mask):
+    levels = [b + 1 for b in base] if missing else list(base)
+    strides = [int(np.prod(levels[k + 1:])) for k in range(len(levels))]
+    code = np.zeros(digits[0][0].shape, dtype=np.int64)
+    per_var = []
+    for (D, ma, mb), L, st in zip(digits, base, strides):
+        D = D.astype(np.int64)
+        if missing:
+            D = np.where(ma[:, None] | mb[None, :], L, D)
+        code += D * st
+        per_var.append([np.nonzero(((D == l) & pair_mask).ravel())[0].astype(np.int64) for l in range(1, L)])
+    code = np.where(pair_mask, code, -1).ravel()
+    n_pat = int(np.prod(levels))
+    order = np.argsort(code, kind="stable")  # ids ascending within a code
+    bounds = np.searchsorted(code[order], np.arange(1, n_pat + 1))
+    indices = [order[bounds[c - 1]:bounds[c]].astype(np.int64) for c in range(1, n_pat)]
+    return levels, per_var, indices
+
+
+def comparison_fit_full(cols_fuzzy_A, cols_fuzzy_B, cols_exact_A=(), cols_exact_B=(), cols_num_A=(), cols_num_B=(),
+                        d2=0.0, d1=1.0, missing=False, p=0.1, lower_thr=0.88, upper_thr=0.94):
+    """Linkage with fuzzy, exact and numeric variables (in that pattern order)
+    and, with ``missing``, a last level per variable for pairs where either
+    value is missing (pd.isna); missing values are compared with nothing.
+    Brute force over every record pair. Returns (levels, per-variable level
+    lists, Indices, Counts)."""
+    n_A, n_B, digits, base = _digits_link(cols_fuzzy_A, cols_fuzzy_B, cols_exact_A, cols_exact_B, cols_num_A,
+                                          cols_num_B, d2, d1, missing, p, lower_thr, upper_thr)
+    levels, per_var, indices = _patterns_from_digits(digits, base, missing, np.ones((n_A, n_B), bool))
+    counts = [len(x) for x in indices]
+    return levels, per_var, indices, np.concatenate([[n_A * n_B - sum(counts)], counts]).astype(np.int64)
+
+
+def deduplication_fit_full(cols_fuzzy, cols_exact=(), cols_num=(), d2=0.0, d1=1.0, missing=False, p=0.1,
+                           lower_thr=0.88, upper_thr=0.94):
+    """Deduplication with fuzzy, exact and numeric variables and optional
+    missing levels (pairs i > j, ids i * n + j; Counts[0] includes the n
+    self-pairs as deduplication.py:825 does). Fuzzy levels come from
+    jaro_winkler_dedup over the non-missing values (orientation and forced
+    diagonal as deduplication.py:150-190)."""
+    n = len((list(cols_fuzzy) + list(cols_exact) + list(cols_num))[0])
+    digits, base = [], []
+    for a in cols_fuzzy:
+        a = np.asarray(a, dtype=object)
+        m = _isna(a) if missing else np.zeros(n, bool)
+        r = np.nonzero(~m)[0]
+        D = np.zeros((n, n), np.int8)
+        if len(r) > 1:
+            l1, l2 = jaro_winkler_dedup(a[r], p, lower_thr, upper_thr)
+            for lvl, ids in ((1, l1), (2, l2)):
+                ids = np.asarray(ids, dtype=np.int64)
+                D[r[ids // len(r)], r[ids % len(r)]] = lvl
+        digits.append((D, m, m))
+        base.append(3)
+    for a in cols_exact:
+        a = np.asarray(a, dtype=object)
+        m = _isna(a) if missing else np.zeros(n, bool)
+        ids = np.full(n, -1, dtype=np.int64)
+        if (~m).any():
+            _, inv = np.unique(a[~m], return_inverse=True)
+            ids[~m] = np.asarray(inv).reshape(-1)
+        D = ((ids[:, None] == ids[None, :]) & (ids[:, None] >= 0)).astype(np.int8)
+        digits.append((D, m, m))
+        base.append(2)
+    for a in cols_num:
+        x = _as_float(a)
+        D = numeric_level_matrix(x, x, d2, d1)
+        m = np.isnan(x) if missing else np.zeros(n, bool)
+        digits.append((D, m, m))
+        base.append(3)
+    lower = np.tril(np.ones((n, n), bool), -1)
+    levels, per_var, indices = _patterns_from_digits(digits, base, missing, lower)
+    counts = [len(x) for x in indices]
+    return levels, per_var, indices, np.concatenate([[n * (n + 1) // 2 - sum(counts)], counts]).astype(np.int64)

@@ -316,3 +316,95 @@ def test_group_strings_order_and_rows(oracle):
     for u, s0, s1 in zip(uniq, starts[:-1], starts[1:]):
         assert rows[s0:s1].tolist() == [i for i, v in enumerate(vals) if v.encode() == u]
     assert oracle.group_strings([])[0] == []
+
+
+# ---------------------- numeric distance and missing levels -------------------
+def test_numeric_levels_pins(oracle):
+    """Numeric-distance levels: with d2 = d1 = 0 they are the exact-equality
+    pairs of oracle.exact / oracle.exact_dedup; on integer data (exact in
+    float64) level >= 1 iff |x - y| <= d1 and level 2 iff |x - y| <= d2, by
+    an independent loop over pairs; NaN pairs with nothing; the relation is
+    symmetric."""
+    rng = np.random.default_rng(3)
+    xa = rng.integers(0, 40, 150).astype(np.float64)
+    xb = rng.integers(0, 40, 170).astype(np.float64)
+    l1, l2 = oracle.numeric_link(xa, xb, 0.0, 0.0)
+    assert len(l1) == 0
+    np.testing.assert_array_equal(l2, oracle.exact(xa.astype(np.int64), xb.astype(np.int64))[0])
+    d1, d2 = oracle.numeric_dedup(xa, 0.0, 0.0)
+    assert len(d1) == 0
+    np.testing.assert_array_equal(d2, oracle.exact_dedup(xa.astype(np.int64))[0])
+    xa[::7] = np.nan
+    xb[::5] = np.nan
+    l1, l2 = oracle.numeric_link(xa, xb, 1.0, 3.0)
+    want1, want2 = [], []
+    for i in range(len(xa)):
+        for j in range(len(xb)):
+            if np.isnan(xa[i]) or np.isnan(xb[j]):
+                continue
+            dist = abs(int(xa[i]) - int(xb[j]))
+            if dist <= 1:
+                want2.append(i * len(xb) + j)
+            elif dist <= 3:
+                want1.append(i * len(xb) + j)
+    np.testing.assert_array_equal(l1, want1)
+    np.testing.assert_array_equal(l2, want2)
+    r1, r2 = oracle.numeric_link(xb, xa, 1.0, 3.0)
+    t = lambda ids: np.sort((ids % len(xa)) * len(xb) + ids // len(xa))
+    np.testing.assert_array_equal(t(r1), l1)
+    np.testing.assert_array_equal(t(r2), l2)
+
+
+def test_missing_levels_pins(oracle):
+    """Missing levels: on data without missing values, Missing=True lists the
+    same pairs as the reference-restating comparison_fit at codes whose
+    digits are unchanged (the extra level is empty); with missing values the
+    pairs whose value is missing on variable v, counted over all patterns,
+    equal the closed form n_A n_B - (n_A - m_A)(n_B - m_B), and the product's
+    P(M) (faster.missing.pair_totals) equals a count by enumeration."""
+    import pandas as pd
+
+    from faster.missing import pair_totals, row_masks
+    from faster.synthetic import add_numeric_and_missing, linkage_tables
+
+    dfa, dfb, truth = linkage_tables(120, 140, seed=4)
+    vf = ["first_name", "last_name"]
+    _, idx_ref, counts_ref = oracle.comparison_fit([dfa[v].to_numpy() for v in vf], [dfb[v].to_numpy() for v in vf],
+                                                      [dfa["house_number"].to_numpy()],
+                                                      [dfb["house_number"].to_numpy()])
+    lv, _, idx, counts = oracle.comparison_fit_full([dfa[v].to_numpy() for v in vf], [dfb[v].to_numpy() for v in vf],
+                                                    [dfa["house_number"].to_numpy()], [dfb["house_number"].to_numpy()],
+                                                    missing=True)
+    assert lv == [4, 4, 3]
+    G_old = np.array(list(itertools.product(range(3), range(3), range(2))))
+    for k, g in enumerate(G_old[1:], start=1):
+        new = g[0] * 12 + g[1] * 3 + g[2]
+        np.testing.assert_array_equal(idx[new - 1], idx_ref[k - 1])
+        assert counts[new] == counts_ref[k]
+    assert counts.sum() == 120 * 140 and counts[0] == counts_ref[0]
+    fa, fb = add_numeric_and_missing(dfa, dfb, truth, seed=4, missing=0.2)
+    lv, _, idx, counts = oracle.comparison_fit_full([fa[v].to_numpy() for v in vf], [fb[v].to_numpy() for v in vf],
+                                                    [fa["house_number"].to_numpy()], [fb["house_number"].to_numpy()],
+                                                    [fa["birth_day"].to_numpy()], [fb["birth_day"].to_numpy()],
+                                                    0.0, 3.0, missing=True)
+    assert lv == [4, 4, 3, 4] and counts.sum() == 120 * 140
+    G = np.array(list(itertools.product(*(range(L) for L in lv))))
+    names = vf + ["house_number", "birth_day"]
+    for v, name in enumerate(names):
+        ma, mb = int(pd.isna(fa[name]).sum()), int(pd.isna(fb[name]).sum())
+        assert counts[G[:, v] == lv[v] - 1].sum() == 120 * 140 - (120 - ma) * (140 - mb), name
+    miss_a = [pd.isna(fa[v]).to_numpy() for v in names]
+    miss_b = [pd.isna(fb[v]).to_numpy() for v in names]
+    ma, mb = row_masks(miss_a, 120), row_masks(miss_b, 140)
+    P = pair_totals(ma, mb, False)
+    enum = {}
+    for x in ma.tolist():
+        for y in mb.tolist():
+            enum[x | y] = enum.get(x | y, 0) + 1
+    assert P == enum
+    Pd = pair_totals(ma, ma, True)
+    enum = {}
+    for i in range(120):
+        for j in range(i):
+            enum[int(ma[i] | ma[j])] = enum.get(int(ma[i] | ma[j]), 0) + 1
+    assert Pd == enum
