@@ -183,7 +183,9 @@ def fit(counts, levels, theta0, method="lbfgs", max_iter=500, tol=1e-14):
 
     'lbfgs': torch L-BFGS (strong-Wolfe line search) on LL / N, then Newton
     polishing steps with the closed-form Hessian; 'newton': Newton steps only
-    (from a good start); 'gd': gradient ascent with backtracking.
+    (from a good start); 'gd': gradient descent on -LL / N with
+    Barzilai-Borwein step lengths, stopping when the largest gradient entry
+    is below ``tol``.
     Returns (theta, LL, iterations)."""
     c = _dev_counts(counts)
     lv = tuple(int(x) for x in levels)
@@ -214,25 +216,35 @@ def fit(counts, levels, theta0, method="lbfgs", max_iter=500, tol=1e-14):
         opt.step(closure)
         iters = opt.state[opt._params[0]].get("n_iter", 0)
     elif method == "gd":
-        step = 1.0
+        # Gradient descent: the direction is always the negative gradient; the
+        # step length is the Barzilai-Borwein one, s's / s'y from the last step
+        # and gradient change, kept only when a nonmonotone Armijo test
+        # against the largest of the last 10 objective values passes, halved
+        # otherwise (Raydan, SIAM J. Optim. 7 (1997) 26-33).
         f = objective(x)
+        g, = torch.autograd.grad(f, x)
+        recent = [f.item()]
+        step = 1.0 / max(1.0, float(g.abs().max()))
         for it in range(max_iter):
-            g, = torch.autograd.grad(f, x)
+            gg = float(g.dot(g))
+            ref = max(recent[-10:])
             while True:
                 xn = (x - step * g).detach().requires_grad_(True)
                 fn = objective(xn)
-                if fn.item() <= f.item() - 0.25 * step * float(g.dot(g)):
+                if fn.item() <= ref - 1e-4 * step * gg or step < 1e-30:
                     break
                 step *= 0.5
-                if step < 1e-12:
-                    break
-            if abs(f.item() - fn.item()) < tol * max(1.0, abs(f.item())):
-                x, f = xn, fn
-                iters = it + 1
-                break
-            x, f = xn, fn
-            step *= 2.0
+            gn, = torch.autograd.grad(fn, xn)
+            sv = (xn - x).detach()
+            yv = (gn - g).detach()
+            sy = float(sv.dot(yv))
+            step = float(sv.dot(sv)) / sy if sy > 0 else 1.0
+            step = min(max(step, 1e-12), 1e12)
+            x, f, g = xn, fn, gn
+            recent.append(f.item())
             iters = it + 1
+            if float(g.abs().max()) < tol:
+                break
     # Newton polishing with the closed-form Hessian (also the whole 'newton' method).
     if method in ("lbfgs", "newton"):
         for _ in range(20 if method == "lbfgs" else max_iter):
@@ -240,7 +252,10 @@ def fit(counts, levels, theta0, method="lbfgs", max_iter=500, tol=1e-14):
             f = -objective(xd) * N
             g, = torch.autograd.grad(f, xd)
             H = fs_hessian(full(xd.detach()), c, lv).cpu()[idx][:, idx]
-            dx = torch.linalg.solve(H, -g.detach().cpu())
+            try:
+                dx = torch.linalg.solve(H, -g.detach().cpu())
+            except RuntimeError:  # singular: a parameter on the boundary
+                break
             xn = (xd.detach() + dx)
             if float(fs_loglik(full(xn), c, lv)) < float(f) - 1e-9 * abs(float(f)):
                 break

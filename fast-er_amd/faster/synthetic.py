@@ -62,27 +62,38 @@ def perturb_all(rng, strings, min_edits, max_edits):
     return out
 
 
-def add_numeric_and_missing(dfa, dfb, truth, seed=0, missing=0.05, days=27759, p_same=0.8, max_shift=3):
-    """Extends linkage tables with a numeric variable and missing values.
+def add_numeric_and_missing(dfa, dfb, truth, seed=0, missing=0.05, days=27759, p_same=0.8, max_shift=3,
+                            p_error=0.02):
+    """Extends linkage tables with a numeric variable, entry errors and
+    missing values.
 
     ``birth_day`` is a day number uniform over ``days`` days (1930-01-01 to
     2005-12-31); a copied B row keeps its A row's day with probability
-    ``p_same`` and is otherwise shifted by 1..``max_shift`` days either way.
-    Then every value of every compared column (the string variables,
-    ``house_number`` and ``birth_day``) is missing with probability
-    ``missing``, independently (strings become None, numbers NaN, so
-    ``house_number`` becomes float64). Draws come from their own generator, so
-    the other columns stay those of ``linkage_tables``."""
+    ``p_same``, is shifted by 1..``max_shift`` days either way otherwise,
+    except that with probability ``p_error`` the shift is 4..60 days (an
+    entry error). With probability ``p_error`` a copied B row's house number
+    is replaced by a uniform one. Then every value of every compared column
+    (the string variables, ``house_number`` and ``birth_day``) is missing with
+    probability ``missing``, independently (strings become None, numbers NaN,
+    so ``house_number`` becomes float64). Draws come from their own
+    generator, so the other columns stay those of ``linkage_tables``."""
     rng = np.random.default_rng(seed + 7919)
     n_a, n_b = len(dfa), len(dfb)
     a_day = rng.integers(0, days, n_a).astype(np.float64)
     b_day = rng.integers(0, days, n_b).astype(np.float64)
     cp = np.nonzero(truth >= 0)[0]
-    shift = rng.integers(1, max_shift + 1, len(cp)) * rng.choice([-1, 1], len(cp))
-    keep = rng.random(len(cp)) < p_same
+    sign = rng.choice([-1, 1], len(cp))
+    shift = rng.integers(1, max_shift + 1, len(cp))
+    far = rng.random(len(cp)) < p_error
+    shift = np.where(far, rng.integers(max_shift + 1, 61, len(cp)), shift) * sign
+    keep = (rng.random(len(cp)) < p_same) & ~far
     b_day[cp] = a_day[truth[cp]] + np.where(keep, 0, shift)
     dfa = dfa.copy()
     dfb = dfb.copy()
+    wrong = cp[rng.random(len(cp)) < p_error]
+    hn = dfb["house_number"].to_numpy().copy()
+    hn[wrong] = rng.integers(1, 10000, len(wrong))
+    dfb["house_number"] = hn
     dfa["birth_day"] = a_day
     dfb["birth_day"] = b_day
     for df in (dfa, dfb):

@@ -142,15 +142,8 @@ N = 1_000_000
 
 
 @pytest.fixture(scope="module")
-def config5():
-    import faster
-
-    fa, fb = _tables(N, N, 0, 0.05)
-    print("1M x 1M with numeric + missing levels", flush=True)
-    comp = faster.Comparison(fa, fb, VF, VF, ["house_number"], ["house_number"], Vars_Numeric_A=["birth_day"],
-                             Vars_Numeric_B=["birth_day"])
-    comp.fit(Numeric_Dist=(0.0, 3.0), Missing=True)
-    return fa, fb, comp
+def config5(config5_fit):
+    return config5_fit
 
 
 def test_config5_histogram_invariants(config5):
