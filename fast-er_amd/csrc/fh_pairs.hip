@@ -319,6 +319,10 @@ struct MergeArgs {
   uint32_t imp_stride[kMaxVars];
   int n_imp;
   uint64_t id_div;
+  // Level count of every listed variable; a key whose level is 0 or not
+  // below it sets *err (the merge then fails with FH_ERR_ARG).
+  uint32_t nlev[kMaxVars];
+  uint32_t* err;
 };
 
 __device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* a, int64_t lo, int64_t hi, uint64_t x) {
@@ -372,8 +376,10 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
   th[threadIdx.x] = 0;
   for (int i = threadIdx.x; i < tn; i += kThreads) {
     const uint64_t key = src[t0 + i];
+    const uint32_t lvl = (uint32_t)(key & 3ull);
+    if (lvl == 0 || lvl >= m.nlev[v]) atomicOr(m.err, 1u);
     tids[i] = key >> 2;
-    tcode[i] = (uint32_t)(key & 3ull) * m.stride[v];
+    tcode[i] = lvl * m.stride[v];
     tgone[i] = 0;
   }
   __syncthreads();
@@ -459,7 +465,13 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
         if (m.imp_a[q][r1] == m.imp_b[q][r2]) code += m.imp_stride[q];
     }
     codes[t0 + i] = code;
-    if (tile_hist && code != 0xFFFFFFFFu) atomicAdd(&th[code - code_lo], 1u);
+    if (tile_hist && code != 0xFFFFFFFFu) {
+      const uint32_t d = code - code_lo;
+      if (d < (uint32_t)ncodes)
+        atomicAdd(&th[d], 1u);
+      else
+        atomicOr(m.err, 1u);  // only reachable through an invalid level key
+    }
   }
   if (tile_hist) {
     __syncthreads();
@@ -509,6 +521,9 @@ __global__ void __launch_bounds__(kThreads) k_code_scatter(const uint32_t* __res
   for (int r = 0; r < kWaveItems / 64; ++r) {
     const int64_t e = wbase + r * 64 + lane;
     code[r] = e < n ? codes[e] : 0xFFFFFFFFu;
+    // Codes outside the variable's range come only from invalid level keys
+    // (flagged by the code kernel); they are dropped, not indexed.
+    if (code[r] != 0xFFFFFFFFu && code[r] - code_lo >= (uint32_t)ncodes) code[r] = 0xFFFFFFFFu;
     const bool valid = code[r] != 0xFFFFFFFFu;
     const uint32_t d = valid ? code[r] - code_lo : 0u;
     uint64_t peers = __ballot(valid);
@@ -1020,6 +1035,7 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
     FH_REQUIRE(var_sizes[v] >= 0 && var_sizes[v] < (int64_t)0xFFFFFFFF, "fh_pattern_merge: list too long");
     n_patterns *= (uint64_t)var_levels[v];
     FH_REQUIRE(n_patterns < (1ull << 31), "fh_pattern_merge: too many patterns");
+    m.nlev[v] = (uint32_t)var_levels[v];
   }
   n_patterns <<= n_imp;
   FH_REQUIRE(n_patterns < (1ull << 31), "fh_pattern_merge: too many patterns");
@@ -1042,56 +1058,62 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
   Scratch scr(s);
   int64_t nmax = 0;
   for (int v = 0; v < n_vars; ++v) nmax = std::max(nmax, var_sizes[v]);
-  uint32_t *codes = nullptr, *flags = nullptr, *ocodes = nullptr, *ocodes_tmp = nullptr, *tot = nullptr;
-  uint64_t *oids = nullptr, *oids_tmp = nullptr;
-  int64_t* bounds = nullptr;
-  const int code_bits = bits_for(n_patterns - 1);
   auto code_range = [&](int v, uint32_t* lo, uint32_t* hi) {
     *lo = m.stride[v];
     *hi = (v == 0 ? (uint32_t)n_patterns : m.stride[v - 1]) - 1;
   };
-  // Up to 256 codes per variable: the owners' counts per tile come out of
-  // the code kernel and one ranked scatter places them, with the running
-  // output base and the code offsets kept on the device and read back once.
-  // More codes take the general path (flag, compact, radix sort by code),
-  // which reads each variable's owner count back.
-  bool one_pass = true;
+  // Per variable: up to 256 owned codes take the one-pass path (the owners'
+  // counts per tile come out of the code kernel and one ranked scatter places
+  // them at the running output base, which stays on the device); more codes
+  // take the general path (flag, compact, radix sort by code), which reads
+  // the base and its owner count back.
+  std::vector<char> one_pass(n_vars);
+  bool any_one = false, any_general = false;
   for (int v = 0; v < n_vars; ++v) {
     uint32_t lo, hi;
     code_range(v, &lo, &hi);
-    one_pass = one_pass && hi - lo + 1 <= 256;
+    one_pass[v] = hi - lo + 1 <= 256;
+    any_one = any_one || one_pass[v];
+    any_general = any_general || !one_pass[v];
   }
+  uint32_t *codes = nullptr, *flags = nullptr, *ocodes = nullptr, *ocodes_tmp = nullptr, *tot = nullptr,
+           *err = nullptr;
+  uint64_t *oids_tmp = nullptr, *state = nullptr;  // state[0]: owners placed so far
+  int64_t *bounds = nullptr, *tile_bounds = nullptr, *d_offsets = nullptr;
+  uint32_t* tile_hist = nullptr;  // owners per (code, tile), for the one-pass code sort
   FH_TRY(scr.alloc(&codes, (size_t)std::max<int64_t>(1, nmax)));
   FH_TRY(scr.alloc(&ocodes, (size_t)std::max<int64_t>(1, nmax)));
   FH_TRY(scr.alloc(&tot, 1));
-  int64_t* tile_bounds = nullptr;
+  FH_TRY(scr.alloc(&err, 1));
+  FH_TRY(scr.alloc(&state, 1));
+  FH_TRY(scr.alloc(&d_offsets, (size_t)n_patterns + 1));
   FH_TRY(scr.alloc(&tile_bounds, (size_t)((ceil_div(nmax, kMergeTile) + 1) * n_vars)));
-  uint32_t* tile_hist = nullptr;  // owners per (code, tile), for the one-pass code sort
-  if (one_pass) {
-    FH_TRY(scr.alloc(&tile_hist, (size_t)(256 * ceil_div(std::max<int64_t>(1, nmax), kMergeTile))));
-  } else {
+  if (any_one) FH_TRY(scr.alloc(&tile_hist, (size_t)(256 * ceil_div(std::max<int64_t>(1, nmax), kMergeTile))));
+  if (any_general) {
     FH_TRY(scr.alloc(&flags, (size_t)std::max<int64_t>(1, nmax)));
     FH_TRY(scr.alloc(&ocodes_tmp, (size_t)std::max<int64_t>(1, nmax)));
     FH_TRY(scr.alloc(&oids_tmp, (size_t)std::max<int64_t>(1, nmax)));
     FH_TRY(scr.alloc(&bounds, (size_t)n_patterns + 1));
   }
-  int64_t base = 0;
-  if (one_pass) {
-    uint64_t* state = nullptr;  // [0] owners placed so far
-    int64_t* d_offsets = nullptr;
-    FH_TRY(scr.alloc(&state, 1));
-    FH_TRY(scr.alloc(&d_offsets, (size_t)n_patterns + 1));
-    FH_TRY(fill_u64(state, 0, 1, s));
-    for (int v = n_vars - 1; v >= 0; --v) {
-      const int64_t n = var_sizes[v];
-      uint32_t code_lo, code_hi;
-      code_range(v, &code_lo, &code_hi);
-      const uint32_t ncodes = code_hi - code_lo + 1;
-      const int64_t ntiles = std::max<int64_t>(1, ceil_div(n, kMergeTile));
+  m.err = err;
+  FH_TRY(fill_u64(state, 0, 1, s));
+  FH_TRY(fill_u32(err, 0, 1, s));
+  const int code_bits = bits_for(n_patterns - 1);
+  // Codes owned by variable v lie in [stride_v, stride_{v-1}); the output is
+  // pattern-major, so variables are processed from the last to the first.
+  for (int v = n_vars - 1; v >= 0; --v) {
+    const int64_t n = var_sizes[v];
+    uint32_t code_lo, code_hi;
+    code_range(v, &code_lo, &code_hi);
+    const uint32_t ncodes = code_hi - code_lo + 1;
+    const int64_t ntiles = std::max<int64_t>(1, ceil_div(n, kMergeTile));
+    if (n > 0) {
+      hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div((ntiles + 1) * n_vars, kThreads)), dim3(kThreads), 0,
+                         s, m, v, ntiles, tile_bounds);
+      FH_HIP(hipGetLastError());
+    }
+    if (one_pass[v]) {
       if (n > 0) {
-        hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div((ntiles + 1) * n_vars, kThreads)), dim3(kThreads),
-                           0, s, m, v, ntiles, tile_bounds);
-        FH_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_pattern_codes, dim3((unsigned)ntiles), dim3(kThreads), 0, s, m, v,
                            (const int64_t*)tile_bounds, codes, code_lo, (int)ncodes, tile_hist);
         FH_HIP(hipGetLastError());
@@ -1106,63 +1128,69 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
       hipLaunchKernelGGL(k_code_offsets, dim3((unsigned)ceil_div(ncodes, kThreads)), dim3(kThreads), 0, s, ocodes,
                          (const uint32_t*)tot, (const uint64_t*)state, capacity, code_lo, code_hi, d_offsets);
       FH_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_advance_base, dim3(1), dim3(64), 0, s, state, (const uint32_t*)tot);
-      FH_HIP(hipGetLastError());
-    }
-    uint64_t h_base = 0;
-    const uint32_t first = m.stride[n_vars - 1];
-    FH_HIP(hipMemcpyAsync(pattern_offsets + first, d_offsets + first, (size_t)(n_patterns - first) * sizeof(int64_t),
-                          hipMemcpyDeviceToHost, s));
-    FH_HIP(hipMemcpyAsync(&h_base, state, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-    FH_HIP(hipStreamSynchronize(s));
-    base = (int64_t)h_base;
-  } else {
-  // Codes owned by variable v lie in [stride_v, stride_{v-1}); the output is
-  // pattern-major, so variables are processed from the last to the first.
-  for (int v = n_vars - 1; v >= 0; --v) {
-    const int64_t n = var_sizes[v];
-    uint32_t code_lo, code_hi;
-    code_range(v, &code_lo, &code_hi);
-    if (n == 0) {
-      for (uint32_t c = code_lo; c <= code_hi; ++c) pattern_offsets[c] = base;
-      continue;
-    }
-    const int64_t ntiles = ceil_div(n, kMergeTile);
-    hipLaunchKernelGGL(k_tile_bounds, dim3((unsigned)ceil_div((ntiles + 1) * n_vars, kThreads)), dim3(kThreads), 0,
-                       s, m, v, ntiles, tile_bounds);
-    FH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_pattern_codes, dim3((unsigned)ntiles), dim3(kThreads), 0, s, m, v,
-                       (const int64_t*)tile_bounds, codes, code_lo, 0, (uint32_t*)nullptr);
-    FH_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_owner_flags, dim3(grid_for(n)), dim3(kThreads), 0, s, codes, n, flags);
-    FH_HIP(hipGetLastError());
-    FH_TRY(exclusive_scan_u32(flags, flags, n, tot, s));
-    uint32_t owners = 0;
-    FH_HIP(hipMemcpyAsync(&owners, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    FH_HIP(hipStreamSynchronize(s));
-    if (base + (int64_t)owners <= capacity && owners > 0) {
-      oids = reinterpret_cast<uint64_t*>(ids + base);
-      hipLaunchKernelGGL(k_owner_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, m.lists[v], codes, n, flags,
-                         ocodes_tmp, oids_tmp);
-      FH_HIP(hipGetLastError());
-      FH_TRY(radix_sort_u32_into(ocodes_tmp, ocodes, oids_tmp, oids, owners, 0, code_bits, s));
-      const uint32_t ncodes = code_hi - code_lo + 1;
-      hipLaunchKernelGGL(k_code_bounds, dim3((unsigned)ceil_div(ncodes, kThreads)), dim3(kThreads), 0, s, ocodes,
-                         (int64_t)owners, code_lo, code_hi, bounds);
-      FH_HIP(hipGetLastError());
-      std::vector<int64_t> hb(ncodes);
-      FH_HIP(hipMemcpyAsync(hb.data(), bounds, ncodes * sizeof(int64_t), hipMemcpyDeviceToHost, s));
-      FH_HIP(hipStreamSynchronize(s));
-      for (uint32_t c = 0; c < ncodes; ++c) pattern_offsets[code_lo + c] = base + hb[c];
     } else {
-      for (uint32_t c = code_lo; c <= code_hi; ++c) pattern_offsets[c] = base;
+      uint64_t h_base = 0;
+      FH_HIP(hipMemcpyAsync(&h_base, state, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+      FH_HIP(hipStreamSynchronize(s));
+      const int64_t base = (int64_t)h_base;
+      uint32_t owners = 0;
+      if (n > 0) {
+        hipLaunchKernelGGL(k_pattern_codes, dim3((unsigned)ntiles), dim3(kThreads), 0, s, m, v,
+                           (const int64_t*)tile_bounds, codes, code_lo, 0, (uint32_t*)nullptr);
+        FH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_owner_flags, dim3(grid_for(n)), dim3(kThreads), 0, s, codes, n, flags);
+        FH_HIP(hipGetLastError());
+        FH_TRY(exclusive_scan_u32(flags, flags, n, tot, s));
+        FH_HIP(hipMemcpyAsync(&owners, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        FH_HIP(hipStreamSynchronize(s));
+      } else {
+        FH_TRY(fill_u32(tot, 0, 1, s));
+      }
+      if (base + (int64_t)owners <= capacity && owners > 0) {
+        uint64_t* oids = reinterpret_cast<uint64_t*>(ids + base);
+        hipLaunchKernelGGL(k_owner_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, m.lists[v], codes, n, flags,
+                           ocodes_tmp, oids_tmp);
+        FH_HIP(hipGetLastError());
+        FH_TRY(radix_sort_u32_into(ocodes_tmp, ocodes, oids_tmp, oids, owners, 0, code_bits, s));
+        hipLaunchKernelGGL(k_code_bounds, dim3((unsigned)ceil_div(ncodes, kThreads)), dim3(kThreads), 0, s, ocodes,
+                           (int64_t)owners, code_lo, code_hi, bounds);
+        FH_HIP(hipGetLastError());
+        std::vector<int64_t> hb(ncodes);
+        FH_HIP(hipMemcpyAsync(hb.data(), bounds, ncodes * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        FH_HIP(hipStreamSynchronize(s));
+        for (uint32_t c = 0; c < ncodes; ++c) pattern_offsets[code_lo + c] = base + hb[c];
+      } else {
+        for (uint32_t c = code_lo; c <= code_hi; ++c) pattern_offsets[c] = base;
+      }
     }
-    base += owners;
+    hipLaunchKernelGGL(k_advance_base, dim3(1), dim3(64), 0, s, state, (const uint32_t*)tot);
+    FH_HIP(hipGetLastError());
   }
+  // One readback: the final base, the error flag and the one-pass offsets.
+  uint64_t h_base = 0;
+  uint32_t h_err = 0;
+  std::vector<int64_t> d_off_host;
+  if (any_one) {
+    d_off_host.resize((size_t)n_patterns + 1);
+    FH_HIP(hipMemcpyAsync(d_off_host.data(), d_offsets, ((size_t)n_patterns + 1) * sizeof(int64_t),
+                          hipMemcpyDeviceToHost, s));
+  }
+  FH_HIP(hipMemcpyAsync(&h_base, state, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipMemcpyAsync(&h_err, err, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  if (h_err) {
+    set_error("fh_pattern_merge: a level key has level 0 or a level not below its variable's level count");
+    return FH_ERR_ARG;
+  }
+  for (int v = 0; v < n_vars; ++v) {
+    if (!one_pass[v]) continue;
+    uint32_t lo, hi;
+    code_range(v, &lo, &hi);
+    for (uint32_t c = lo; c <= hi; ++c) pattern_offsets[c] = d_off_host[c];
   }
   pattern_offsets[0] = 0;
-  pattern_offsets[n_patterns] = base;
-  *n_union = base;
+  pattern_offsets[n_patterns] = (int64_t)h_base;
+  *n_union = (int64_t)h_base;
   return FH_OK;
 }
 

@@ -232,9 +232,10 @@ def test_comparison_fit_small(oracle):
 
 
 def test_comparison_fit_many_variables(oracle):
-    """Six fuzzy variables (729 patterns): the first variable owns 486 codes
-    and takes the general owner path (flag, compact, radix sort by code);
-    the others own at most 162 and take the one-pass ranked scatter."""
+    """Six fuzzy variables (729 patterns) in one merge that mixes both owner
+    paths: the first variable owns 486 codes and takes the general path
+    (flag, compact, radix sort by code); the others own at most 162 and take
+    the one-pass ranked scatter."""
     import faster
 
     dfa, dfb, _ = _frames(700, 800, 5)
@@ -252,6 +253,91 @@ def test_comparison_fit_many_variables(oracle):
         _eq_lists(comp.Levels[v], per_var[v], f"var {v}")
     _eq_lists(comp.Indices, indices, "Indices")
     np.testing.assert_array_equal(comp.Counts, counts)
+
+
+@pytest.mark.parametrize("empty", ["general", "one_pass"])
+def test_merge_with_empty_lists(oracle, empty):
+    """Empty level lists in both owner paths: a fuzzy variable whose two sides
+    share no character (no pair above the threshold) as the variable with
+    486 codes (general path) or as a one-pass variable, and an exact
+    variable with no equal pair."""
+    import faster
+
+    dfa, dfb, _ = _frames(300, 350, 12)
+    base = ["first_name", "last_name", "street"]
+    for df in (dfa, dfb):
+        for v in base:
+            df[v + "_r"] = df[v].str[::-1]
+    dfa["none"] = ["x" * (1 + i % 5) for i in range(len(dfa))]
+    dfb["none"] = ["y" * (1 + i % 5) for i in range(len(dfb))]
+    dfa["hn"] = np.arange(len(dfa))
+    dfb["hn"] = np.arange(len(dfb)) + 10_000
+    vf = ["none"] + base + ["last_name_r", "street_r"] if empty == "general" else \
+        base + ["first_name_r", "none", "street_r"]
+    comp = faster.Comparison(dfa, dfb, vf, vf, ["hn"], ["hn"])
+    comp.fit()
+    per_var, indices, counts = oracle.comparison_fit([dfa[v].to_numpy() for v in vf], [dfb[v].to_numpy() for v in vf],
+                                                     [dfa["hn"].to_numpy()], [dfb["hn"].to_numpy()])
+    k = vf.index("none")
+    assert all(len(x) == 0 for x in per_var[k]) and len(per_var[-1][0]) == 0
+    _eq_lists(comp.Indices, indices, "Indices")
+    np.testing.assert_array_equal(comp.Counts, counts)
+
+
+def test_pattern_merge_capacity_and_bad_keys():
+    """fh_pattern_merge through the C ABI: with a capacity below the union it
+    returns FH_OK, reports the full union size and writes nothing past the
+    capacity (for a one-pass and a general-path merge); a level key with
+    level 0, or a level not below its variable's level count, is rejected
+    with FH_ERR_ARG."""
+    import ctypes
+
+    from faster import _lib
+
+    L = _lib.load()
+    rng = np.random.default_rng(4)
+    dev = torch.device("cuda")
+
+    def lists(nv, levels):
+        out = []
+        for v in range(nv):
+            ids = np.unique(rng.integers(0, 5000, 1500))
+            lv = rng.integers(1, levels[v], len(ids))
+            out.append(torch.as_tensor(ids * 4 + lv, dtype=torch.int64, device=dev))
+        return out
+
+    def merge(keys, levels, capacity, total):
+        nv = len(keys)
+        ptrs = (ctypes.c_void_p * nv)(*[k.data_ptr() for k in keys])
+        sizes = (ctypes.c_int64 * nv)(*[k.numel() for k in keys])
+        lv = (ctypes.c_int32 * nv)(*levels)
+        ids = torch.full((total,), -7, dtype=torch.int64, device=dev)
+        n_pat = int(np.prod(levels))
+        offs = (ctypes.c_int64 * (n_pat + 1))()
+        nu = ctypes.c_int64(-1)
+        st = L.fh_pattern_merge(nv, ptrs, sizes, lv, _lib.ptr(ids), ctypes.c_int64(capacity), offs, ctypes.byref(nu),
+                                _lib.stream())
+        torch.cuda.synchronize()
+        return st, nu.value, ids.cpu().numpy(), list(offs)
+
+    for levels in ([3, 3, 2], [3] * 6):
+        keys = lists(len(levels), levels)
+        total = sum(k.numel() for k in keys)
+        st, n_union, ids_full, offs_full = merge(keys, levels, total, total)
+        assert st == 0 and n_union == offs_full[-1]
+        union = np.unique(np.concatenate([k.cpu().numpy() >> 2 for k in keys]))
+        assert n_union == len(union)
+        cap = n_union // 2
+        st, nu2, ids_small, _ = merge(keys, levels, cap, total)
+        assert st == 0 and nu2 == n_union
+        assert np.all(ids_small[cap:] == -7)
+        bad = [k.clone() for k in keys]
+        bad[1][5] = (bad[1][5] >> 2) * 4  # level 0
+        assert merge(bad, levels, total, total)[0] == 2
+        assert b"level" in L.fh_last_error()
+        bad = [k.clone() for k in keys]
+        bad[-1][3] = (bad[-1][3] >> 2) * 4 + levels[-1]  # level == level count
+        assert merge(bad, levels, total, total)[0] == 2
 
 
 def test_deduplication_fit(oracle):
