@@ -307,7 +307,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   // Work accounting (host arithmetic over the length histograms).
   if (stats) {
     stats->alphabet = K;
-    double kern = 0, pruned = 0, ops = 0;
+    double kern = 0, pruned = 0, ops = 0, frozen = 0;
     for (int la = 1; la <= kLongLen; ++la)
       for (int lb = 1; lb <= kLongLen; ++lb) {
         double pairs;
@@ -327,10 +327,12 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
         }
         kern += pairs;
         ops += pairs * kind_pair_ops(kind_of_len(lb), la);
+        frozen += pairs * (8.0 * la + 24.0);
       }
     stats->pairs_kernel = (int64_t)kern;
     stats->pairs_pruned = (int64_t)pruned;
     stats->alg_lane_ops = bound ? 0.0 : ops;
+    stats->frozen_model_ops = frozen;
     // One 64-byte dot product (64 multiply-adds) per pair.
     stats->alg_mfma_ops = bound ? kern * 128.0 : 0.0;
     stats->method = method;

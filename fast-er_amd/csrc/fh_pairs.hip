@@ -262,11 +262,16 @@ __global__ void __launch_bounds__(kThreads) k_level_flags(const uint64_t* __rest
     flags[i] = (int)(keys[i] & 3ull) == level ? 1u : 0u;
 }
 
+// Level l's ids go after those of levels 1 .. l - 1, whose counts are
+// tot[0 .. l - 2] (device), so the split needs no host readback per level.
 __global__ void __launch_bounds__(kThreads) k_level_scatter(const uint64_t* __restrict__ keys, int64_t n, int level,
                                                             const uint32_t* __restrict__ pos,
+                                                            const uint32_t* __restrict__ tot,
                                                             int64_t* __restrict__ ids) {
+  int64_t base = 0;
+  for (int l = 1; l < level; ++l) base += tot[l - 1];
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    if ((int)(keys[i] & 3ull) == level) ids[pos[i]] = (int64_t)(keys[i] >> 2);
+    if ((int)(keys[i] & 3ull) == level) ids[base + pos[i]] = (int64_t)(keys[i] >> 2);
 }
 
 // ------------------------ sorted-set membership ----------------------------
@@ -889,7 +894,6 @@ extern "C" int fh_expand_link(const uint64_t* hits, int64_t n_hits, int64_t n_ub
   hipLaunchKernelGGL(k_expand_link, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, hits, n_hits, n_ub,
                      starts_a, rows_a, starts_b, rows_b, n_b, cnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
-  FH_HIP(hipStreamSynchronize(s));
   return FH_OK;
 }
 
@@ -917,7 +921,6 @@ extern "C" int fh_expand_dedup(const uint64_t* hits, int64_t n_hits, int64_t n_u
   hipLaunchKernelGGL(k_expand_dedup, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, hits, n_hits, n_virtual,
                      n_u, diag_level, starts, rows, n, cnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
-  FH_HIP(hipStreamSynchronize(s));
   return FH_OK;
 }
 
@@ -944,7 +947,6 @@ extern "C" int fh_exact_link(const int64_t* group_a, int64_t n_a, const int64_t*
   hipLaunchKernelGGL(k_exact_link, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, group_a, n_a, starts_b,
                      rows_b, n_b, cnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
-  FH_HIP(hipStreamSynchronize(s));
   return FH_OK;
 }
 
@@ -970,7 +972,6 @@ extern "C" int fh_exact_dedup(const int64_t* group, int64_t n, const int64_t* st
   hipLaunchKernelGGL(k_exact_dedup, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, group, n, starts, rows,
                      cnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
-  FH_HIP(hipStreamSynchronize(s));
   return FH_OK;
 }
 
@@ -990,18 +991,21 @@ extern "C" int fh_split_levels(const uint64_t* keys, int64_t n, int n_levels, in
   Scratch scr(s);
   uint32_t *flags = nullptr, *tot = nullptr;
   FH_TRY(scr.alloc(&flags, (size_t)n));
-  FH_TRY(scr.alloc(&tot, 1));
-  int64_t base = 0;
+  FH_TRY(scr.alloc(&tot, 4));
   for (int l = 1; l < n_levels; ++l) {
     hipLaunchKernelGGL(k_level_flags, dim3(grid_for(n)), dim3(kThreads), 0, s, keys, n, l, flags);
     FH_HIP(hipGetLastError());
-    FH_TRY(exclusive_scan_u32(flags, flags, n, tot, s));
-    hipLaunchKernelGGL(k_level_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, keys, n, l, flags, ids + base);
+    FH_TRY(exclusive_scan_u32(flags, flags, n, tot + (l - 1), s));
+    hipLaunchKernelGGL(k_level_scatter, dim3(grid_for(n)), dim3(kThreads), 0, s, keys, n, l, flags,
+                       (const uint32_t*)tot, ids);
     FH_HIP(hipGetLastError());
-    uint32_t c = 0;
-    FH_HIP(hipMemcpyAsync(&c, tot, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    FH_HIP(hipStreamSynchronize(s));
-    base += c;
+  }
+  uint32_t c[4] = {0, 0, 0, 0};
+  FH_HIP(hipMemcpyAsync(c, tot, sizeof(c), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  int64_t base = 0;
+  for (int l = 1; l < n_levels; ++l) {
+    base += c[l - 1];
     level_offsets[l + 1] = base;
   }
   return FH_OK;
@@ -1287,7 +1291,6 @@ extern "C" int fh_numeric_link(const double* values_a, int64_t n_a, const double
   hipLaunchKernelGGL(k_num_link, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, (const int64_t*)win, n_a,
                      (const uint64_t*)perm, n_b, cnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
-  FH_HIP(hipStreamSynchronize(s));
   return FH_OK;
 }
 
@@ -1321,7 +1324,6 @@ extern "C" int fh_numeric_dedup(const double* values, int64_t n, double d2, doub
   hipLaunchKernelGGL(k_num_dedup, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, (const int64_t*)win, nv,
                      (const uint64_t*)perm, n, cnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
-  FH_HIP(hipStreamSynchronize(s));
   return FH_OK;
 }
 

@@ -46,6 +46,7 @@ class JwStats(ctypes.Structure):
         ("pairs_candidates", ctypes.c_int64),
         ("pass_ms", ctypes.c_double),
         ("eval_ms", ctypes.c_double),
+        ("frozen_model_ops", ctypes.c_double),
     ]
 
     def as_dict(self):

@@ -40,12 +40,20 @@ def bits_for(max_value: int) -> int:
     return max(1, int(max_value).bit_length())
 
 
+_hit_rate = [0.0]  # largest hits per value pair seen so far (sizes the next arena)
+
+
 def jw_levels(chars_a, off_a, n_a, chars_b, off_b, n_b, mode, p, lower_thr, upper_thr,
               capacity=None, want_stats=False):
-    """All-pairs JW levels (fh_jw_levels). Returns (hit keys, stats dict or None)."""
+    """All-pairs JW levels (fh_jw_levels). Returns (hit keys, stats dict or None).
+
+    The hit arena is sized from the rows and from the largest hit rate of
+    earlier calls (with 25% headroom), so an overflow, which re-runs the
+    pass with the exact size, is rare."""
     L = _lib.load()
+    pairs = float(n_a) * float(n_b)
     if capacity is None:
-        capacity = max(1 << 20, 4 * (int(n_a) + int(n_b)))
+        capacity = max(1 << 20, 4 * (int(n_a) + int(n_b)), int(1.25 * _hit_rate[0] * pairs))
     stats = _lib.JwStats()
 
     def run(cap):
@@ -58,6 +66,8 @@ def jw_levels(chars_a, off_a, n_a, chars_b, off_b, n_b, mode, p, lower_thr, uppe
         return n.value, hits
 
     need, hits = _retry(run, capacity)
+    if pairs > 0:
+        _hit_rate[0] = max(_hit_rate[0], need / pairs)
     return hits[:need], (stats.as_dict() if want_stats else None)
 
 

@@ -12,7 +12,9 @@
  *     call ends.
  *   - Work is ordered on `stream` (a hipStream_t, e.g. torch's current
  *     stream). Entry points that report a size to the host (n_hits, n_out,
- *     ...) synchronise the stream before returning.
+ *     ...) synchronise the stream once the size is known; the work that
+ *     fills the output may still be running when they return (later work on
+ *     `stream` is ordered after it).
  *   - Output arenas have a capacity. When the result does not fit, the call
  *     still returns FH_OK, reports the required size, and writes nothing
  *     beyond the capacity; the caller grows the arena and calls again.
@@ -76,6 +78,8 @@ typedef struct {
   int64_t pairs_candidates; /* pairs handed to the scoring kernel (bound passed / greedy count passed) */
   double pass_ms;          /* device time of the all-pairs pass kernels alone (HIP events) */
   double eval_ms;          /* device time of the candidate scoring kernels (HIP events) */
+  double frozen_model_ops; /* sum over the evaluated pairs of 8 * len1 + 24: a fixed work model of a
+                              non-packed recurrence (8 ops per string-1 character + 24 per pair) */
 } fh_jw_stats;
 
 /*
