@@ -15,6 +15,14 @@ behaviour relies on a third-party primitive (``np.unique``, CuPy's stable
 ``argsort``/``sort``/``cumsum``), the same primitive is taken from NumPy: the
 outputs are sorted sets, so any correct sort gives identical lists.
 
+The numeric-distance and missing-level functions at the end of the module
+(``numeric_link``, ``numeric_dedup``, ``comparison_fit_full``,
+``deduplication_fit_full``) have no reference counterpart; they are brute
+force over every record pair, and tests/test_oracle.py pins them (equality
+at distance 0 equals ``exact``, a pair loop on integer data, symmetry, the
+embedding of the reference-restating ``comparison_fit`` when nothing is
+missing, closed-form missing totals).
+
 Documented deviations from the reference:
 
 * EM runs entirely in FP64. The reference accumulates the E-step log-sum in a
