@@ -38,13 +38,6 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-class _HostGroups:
-    """Unique values of one side on the host (chars, offsets, count)."""
-
-    def __init__(self, chars, offsets, n_unique):
-        self.chars, self.offsets, self.n_unique = chars, offsets, n_unique
-
-
 INT8_MFMA_PEAK_OPS = 5.0e15  # dense int8 MFMA, MI355X
 
 
@@ -267,9 +260,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        groups = [tuple(_HostGroups(g.chars.cpu().numpy(), g.offsets.cpu().numpy(), g.n_unique)
-                        for g in (group_packed(a), group_packed(b))) for a, b in inputs.fuzzy]
-        cpu = cpu_baseline(groups, n, args.cpu_seconds)
+        cpu = cpu_baseline(dfa, dfb, vf, ["house_number"], args.cpu_seconds)
     del inputs
     torch.cuda.empty_cache()
 
@@ -500,14 +491,10 @@ def bench_blocked(args):
 def bench_realdata(args):
     """The reference's example data (100,000 x 100,000 voter records, linked as
     its notebook does: last_name, first_name, house_number, street_name fuzzy,
-    birth_year exact). One step = the device stage of Comparison.fit from
-    HBM-resident grouped columns."""
+    birth_year exact). One step = Comparison.fit_device from HBM-resident
+    packed columns (device grouping included)."""
     import numpy as np
     import torch
-
-    from faster import _ops
-    from faster.comparison import exact_keys_link, fuzzy_keys_link
-    from faster.encode import group_exact_link, group_strings_device
 
     sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
     import make_realdata
@@ -515,25 +502,29 @@ def bench_realdata(args):
     dfa, dfb, _ = make_realdata.load()
     n_a, n_b = len(dfa), len(dfb)
     vf = ["last_name", "first_name", "house_number", "street_name"]
-    groups = [(group_strings_device(dfa[v].to_numpy()), group_strings_device(dfb[v].to_numpy())) for v in vf]
-    ex = group_exact_link(dfa["birth_year"].to_numpy(), dfb["birth_year"].to_numpy())
-    ga_ex, sb_ex, rb_ex = ex.to_device()
+    import faster
+
+    inputs = faster.Comparison(dfa, dfb, vf, vf, ["birth_year"], ["birth_year"]).device_inputs()
     torch.cuda.synchronize()
 
-    def step(stats=None):
-        keys = [fuzzy_keys_link(a, b, 0.1, 0.88, 0.94, stats) for a, b in groups]
-        keys.append(exact_keys_link(ga_ex, sb_ex, rb_ex, ex.n_groups, n_b))
-        return keys, _ops.pattern_merge(keys, [3, 3, 3, 3, 2])
+    def step(stats_out=None):
+        c = faster.Comparison(dfa, dfb, vf, vf, ["birth_year"], ["birth_year"])
+        c.fit_device(inputs, 0.1, 0.88, 0.94, Exact_Mode="list")
+        if stats_out is not None:
+            stats_out.extend(c.Stats)
+        return c
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     times, stats_all = [], []
+    comp = None
     for k in range(args.steps):
         st = []
+        comp = None
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        keys, (ids, offs, n_union) = step(st)
+        comp = step(st)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         stats_all.append(st)
@@ -542,6 +533,8 @@ def bench_realdata(args):
     comps = float(n_a) * float(n_b) * len(vf)
     roof = comparison_roofline(stats_all, f"comparison ({args.method}, 4 vars)", None)
     st0 = stats_all[-1]
+    keys = comp._var_keys
+    n_union = int(comp._pattern_ids.numel())
     e2e = end_to_end_realdata(dfa, dfb, vf)
     line = {
         "metric": "JW pair-comparisons/s, reference example data 100k x 100k, 4 string vars",
@@ -549,7 +542,8 @@ def bench_realdata(args):
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u8", "data": "reference example voter records (example/df_{A,B}_100000.Rdata)",
         "config": {"workload": "linkage of the reference's 100,000 x 100,000 example records as its notebook does: "
-                               "4 fuzzy vars + birth_year exact", "method": args.method},
+                               "4 fuzzy vars + birth_year exact; step = Comparison.fit from device-resident packed "
+                               "columns", "method": args.method},
         "roofline": roof, "cpu_baseline": None,
         "extras": {"unique_pairs_per_var": [s["pairs_logical"] for s in st0],
                    "comparison_stage_ms_per_var": [s["kernel_ms"] for s in st0],
@@ -607,38 +601,53 @@ def _end_to_end_once(dfa, dfb, vf):
     return t
 
 
-def cpu_baseline(groups, n, seconds):
-    """The oracle's C restatement on all host cores, on a contiguous slice of
-    unique A values of each string variable against all unique B values
-    (work per A value is uniform under the generator), scaled to
-    comparisons/s."""
+def cpu_baseline(dfa, dfb, vf, ex, seconds):
+    """The oracle's whole Comparison.fit pipeline on the host cores (C
+    Jaro-Winkler over a thread pool, NumPy for unique values, expansion, exact
+    pairs and the pattern merge), run on two contiguous samples of A rows
+    against every B row. The time is fitted as fixed + per-row * rows (the
+    fixed part, mostly the B side's unique values, is paid once per job) and
+    extrapolated to all A rows: work per A row is uniform under the
+    generator."""
     import time as _t
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import numpy as np
-
     import faster_oracle as oracle
 
     threads = oracle.n_threads()
-    # Calibrate on a small slice, then size the sample for ~`seconds`.
-    ga, gb = groups[0]
-    sub = lambda g, r: (g.chars, g.offsets[:r + 1])
+    n_a, n_b = len(dfa), len(dfb)
+
+    def run(rows):
+        sub = dfa.iloc[:rows]
+        t0 = _t.perf_counter()
+        oracle.comparison_fit([sub[v].to_numpy() for v in vf], [dfb[v].to_numpy() for v in vf],
+                              [sub[e].to_numpy() for e in ex], [dfb[e].to_numpy() for e in ex], threads=threads)
+        return _t.perf_counter() - t0
+
+    # Per-row cost estimate from the comparison scan alone (B packed
+    # beforehand), to size the second sample.
+    probe = 16
+    packed_b = [oracle.pack_strings(list(dfb[v].to_numpy())) for v in vf]
     t0 = _t.perf_counter()
-    r0 = max(threads, 8)
-    oracle.jaro_winkler_all_pairs(sub(ga, r0), (gb.chars, gb.offsets), threads=threads)
-    rate = r0 * gb.n_unique / (_t.perf_counter() - t0)
-    rows = max(threads, int(rate * seconds / (3 * gb.n_unique)))
-    log(f"CPU baseline: {threads} threads, ~{rate:.3g} pairs/s, sampling {rows} A values x 3 vars")
-    pairs = 0
-    t0 = _t.perf_counter()
-    for ga, gb in groups:
-        r = min(rows, ga.n_unique)
-        oracle.jaro_winkler_all_pairs(sub(ga, r), (gb.chars, gb.offsets), threads=threads)
-        pairs += r * gb.n_unique
-    dt = _t.perf_counter() - t0
-    return {"value": pairs / dt, "unit": "comparisons/s", "cores": threads, "kind": "port",
-            "sample": f"{rows} unique A values x all {groups[0][1].n_unique} unique B values per string var "
-                      f"(3 vars, {pairs:.3g} JW pairs, {dt:.1f} s); all-pairs level scan only (no expansion/merge)"}
+    for v, pb in zip(vf, packed_b):
+        oracle.jaro_winkler_all_pairs(list(dfa[v].to_numpy()[:probe]), pb, threads=threads)
+    per_row = (_t.perf_counter() - t0) / probe
+    del packed_b
+    r1 = 32
+    r2 = r1 + max(64, int(seconds * 0.5 / max(per_row, 1e-6)))
+    t1 = run(r1)
+    t2 = run(r2)
+    c = max(t2 - t1, 1e-9) / (r2 - r1)
+    fixed = max(t1 - r1 * c, 0.0)
+    full = fixed + c * n_a
+    log(f"CPU baseline: {threads} threads; {r1} rows {t1:.1f} s, {r2} rows {t2:.1f} s -> fixed {fixed:.1f} s + "
+        f"{c * 1e3:.2f} ms per A row -> {full:.0f} s for all {n_a} rows")
+    return {"value": float(n_a) * n_b * len(vf) / full, "unit": "comparisons/s", "cores": threads, "kind": "port",
+            "sample": f"the oracle's Comparison.fit (unique values, all-pairs JW, expansion, exact pairs, pattern merge, "
+                      f"{len(vf)} string vars + {len(ex)} exact) on the first {r1} and {r2} A rows x all {n_b} B rows "
+                      f"({t1:.1f} s and {t2:.1f} s), fitted as {fixed:.1f} s + {c * 1e3:.2f} ms per A row and "
+                      f"extrapolated to {n_a} rows ({full:.0f} s)",
+            "extrapolated_s": full}
 
 
 if __name__ == "__main__":
