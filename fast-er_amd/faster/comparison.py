@@ -18,7 +18,7 @@ import numpy as np
 import pandas as pd
 import torch
 
-from . import _lib, _ops, missing, patterns
+from . import _lib, _ops, concurrent, missing, patterns
 from .encode import (DeviceExactLinkGroups, exact_column, group_exact_link, group_exact_link_device, group_packed,
                      group_strings_device, pack_strings, packed_column_device)
 from .levels import level_of_score
@@ -271,13 +271,21 @@ class Comparison():
         n_a, n_b = inputs.n_a, inputs.n_b
         stats = []
         keys = []
-        for v, (ca, cb) in enumerate(inputs.fuzzy):
-            ga, gb = group_packed(ca), group_packed(cb)
-            if inputs.missing:
-                ra, rb = inputs.fuzzy_rows[v]
-                ga, gb = _remap_rows(ga, ra), _remap_rows(gb, rb)
-                ga.n_rows, gb.n_rows = n_a, n_b
-            keys.append(fuzzy_keys_link(ga, gb, p, Lower_Thr, Upper_Thr, stats))
+        def fuzzy_task(v):
+            def task():
+                ca, cb = inputs.fuzzy[v]
+                ga, gb = group_packed(ca), group_packed(cb)
+                if inputs.missing:
+                    ra, rb = inputs.fuzzy_rows[v]
+                    ga, gb = _remap_rows(ga, ra), _remap_rows(gb, rb)
+                    ga.n_rows, gb.n_rows = n_a, n_b
+                st = []
+                return fuzzy_keys_link(ga, gb, p, Lower_Thr, Upper_Thr, st), st
+            return task
+
+        for k, st in concurrent.run([fuzzy_task(v) for v in range(len(inputs.fuzzy))]):
+            keys.append(k)
+            stats.extend(st)
         if inputs.missing:
             exact = [_exact_groups_link(c, r[0], r[1], r[2], n_a, n_b) for c, r in zip(inputs.exact, inputs.exact_rows)]
         else:

@@ -16,7 +16,7 @@ import numpy as np
 import pandas as pd
 import torch
 
-from . import _lib, _ops, missing, patterns
+from . import _lib, _ops, concurrent, missing, patterns
 from .comparison import _split
 from .encode import (DeviceExactDedupGroups, exact_column, group_exact_dedup, group_exact_dedup_device, group_packed,
                      group_strings_device, packed_column_device)
@@ -153,12 +153,19 @@ class Deduplication():
         n = inputs.n
         stats = []
         keys = []
-        for k, c in enumerate(inputs.fuzzy):
-            g = group_packed(c)
-            if inputs.missing:
-                g.rows = inputs.fuzzy_rows[k][g.rows]
-                g.n_rows = n
-            keys.append(fuzzy_keys_dedup(g, p, Lower_Thr, Upper_Thr, stats))
+        def fuzzy_task(k):
+            def task():
+                g = group_packed(inputs.fuzzy[k])
+                if inputs.missing:
+                    g.rows = inputs.fuzzy_rows[k][g.rows]
+                    g.n_rows = n
+                st = []
+                return fuzzy_keys_dedup(g, p, Lower_Thr, Upper_Thr, st), st
+            return task
+
+        for kk, st in concurrent.run([fuzzy_task(k) for k in range(len(inputs.fuzzy))]):
+            keys.append(kk)
+            stats.extend(st)
         exact = []
         for q, c in enumerate(inputs.exact):
             g = group_exact_dedup_device(c)
