@@ -55,11 +55,18 @@ def comparison_roofline(stats_all, kernel_name, traffic):
         return {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_MFMA_PEAK_OPS / 1e12,
                 "unit": "TOP/s (int8)", "frac": achieved / INT8_MFMA_PEAK_OPS, "traffic": traffic,
                 "kernel": kernel_name, "kernel_ms": kern_ms, "alg_ops": mfma}
-    achieved = lane / (kern_ms / 1e3)
+    # The prefix form skips the recurrence steps of shared prefixes: its
+    # algorithmic work is the per-pair count less those steps.
+    skipped = float(np.mean([sum(s.get("prefix_skipped_ops", 0.0) for s in st) for st in stats_all]))
+    executed = lane - skipped
+    achieved = executed / (kern_ms / 1e3)
     frozen = float(np.mean([sum(s.get("frozen_model_ops", 0.0) for s in st) for st in stats_all]))
     return {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_LANE_OPS_PEAK / 1e12,
             "unit": "Tlane-op/s", "frac": achieved / VALU_LANE_OPS_PEAK, "traffic": traffic,
-            "kernel": kernel_name, "kernel_ms": kern_ms, "alg_ops": lane,
+            "kernel": kernel_name, "kernel_ms": kern_ms, "alg_ops": executed,
+            # Lane-ops of the same pairs when every character step is
+            # computed (the per-lane form's work), at this kernel time.
+            "per_pair_form_ops": lane, "per_pair_form_frac": lane / (kern_ms / 1e3) / VALU_LANE_OPS_PEAK,
             # The work model fixed before the kernel was written (8 lane-ops per
             # string-1 character + 24 per evaluated pair) against 3.93e13
             # lane-ops/s (the FP32 vector rate read as packed): a model of a
@@ -67,6 +74,33 @@ def comparison_roofline(stats_all, kernel_name, traffic):
             "frozen_model": {"ops": frozen, "peak": FROZEN_PEAK / 1e12, "unit": "Tlane-op/s",
                              "achieved": frozen / (kern_ms / 1e3) / 1e12,
                              "frac": frozen / (kern_ms / 1e3) / FROZEN_PEAK}}
+
+
+KERNEL_NAMES = {"bound": "k_jw_bound (int8 MFMA bound + exact candidates)",
+                "tiles": "k_jw_tiles (bit-parallel, one string-1 value per lane)",
+                "prefix": "k_jw_prefix (bit-parallel, string-1 values walked per wave, shared prefixes reused)"}
+
+
+def roofline_steps(step, method, traffic, n_vars, repeats=2):
+    """The comparison kernel's roofline from `repeats` extra steps run with
+    the variables one after another (faster.concurrent.SERIAL): in the timed
+    steps the variables' kernels share the GPU, so their HIP-event times
+    overlap and would not add up to kernel time. Returns (roofline, stats of
+    the last step)."""
+    import torch
+    from faster import concurrent
+
+    stats_all = []
+    concurrent.SERIAL = True
+    try:
+        for _ in range(repeats):
+            st = []
+            step(st)
+            torch.cuda.synchronize()
+            stats_all.append(st)
+    finally:
+        concurrent.SERIAL = False
+    return comparison_roofline(stats_all, f"{KERNEL_NAMES[method]}, {n_vars} vars", traffic), stats_all[-1]
 
 
 def valu_probe(dev):
@@ -127,9 +161,10 @@ def main():
     ap.add_argument("--valu-probe", action="store_true",
                     help="run only the VALU throughput probe of the match recurrence (checks the VALU peak)")
     ap.add_argument("--n", type=int, default=None, help="records per side (default 1M link, 2M dedup)")
-    ap.add_argument("--method", choices=["tiles", "bound"], default="tiles",
-                    help="all-pairs comparison: the bit-parallel match of every pair (tiles, default), or an int8 "
-                         "MFMA bound with exact scoring of the candidates (bound)")
+    ap.add_argument("--method", choices=["prefix", "tiles", "bound"], default="prefix",
+                    help="all-pairs comparison: the bit-parallel match with string-1 values walked per wave and "
+                         "shared prefixes reused (prefix, default), the same match with one string-1 value per lane "
+                         "(tiles), or an int8 MFMA bound with exact scoring of the candidates (bound)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-blocked", action="store_true",
@@ -217,11 +252,8 @@ def main():
     comps = float(n) * float(n) * 3
     value = comps / (ms / 1e3)
 
-    kname = "k_jw_bound (int8 MFMA bound + exact candidates, 3 vars)" if args.method == "bound" else \
-        "k_jw_tiles (bit-parallel, 3 vars)"
-    roof = comparison_roofline(stats_all, kname, measured_traffic(f"link_{args.method}", n))
+    roof, st0 = roofline_steps(step, args.method, measured_traffic(f"link_{args.method}", n), 3)
     kern_ms = roof["kernel_ms"]
-    st0 = stats_all[-1]
     counts = comp.Counts
     level_sizes = [int(k.numel()) for k in comp._var_keys]
     n_union = int(comp._pattern_ids.numel())
@@ -359,7 +391,7 @@ def bench_dedup(args):
         log(f"step {k + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms")
     ms = 1e3 * sum(times) / len(times)
     comps = float(n) * float(n - 1) / 2 * 3
-    roof = comparison_roofline(stats_all, f"k_jw_{'bound' if args.method == 'bound' else 'tiles'} (triangle, 3 vars)",
+    roof = comparison_roofline(stats_all, f"{KERNEL_NAMES[args.method]} (triangle), 3 vars",
                                measured_traffic(f"dedup_{args.method}", n))
     np.random.seed(0)
     est = faster.Estimation(3, 1, counts)
@@ -531,8 +563,7 @@ def bench_realdata(args):
         log(f"step {k + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms")
     ms = 1e3 * float(np.mean(times))
     comps = float(n_a) * float(n_b) * len(vf)
-    roof = comparison_roofline(stats_all, f"comparison ({args.method}, 4 vars)", None)
-    st0 = stats_all[-1]
+    roof, st0 = roofline_steps(step, args.method, None, 4)
     keys = comp._var_keys
     n_union = int(comp._pattern_ids.numel())
     e2e = end_to_end_realdata(dfa, dfb, vf)

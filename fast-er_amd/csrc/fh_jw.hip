@@ -9,16 +9,93 @@
 #include <mutex>
 
 #include "fh_jw_core.h"
+#include "fh_jw_prefix.h"
 
 namespace fh {
 namespace {
-std::atomic<int> g_jw_method{FH_JW_METHOD_TILES};
+std::atomic<int> g_jw_method{FH_JW_METHOD_PREFIX};
+// Survivors per kernel pair of the prefix form: the largest rate seen, decayed
+// by 5% per call (sizes the next call's candidate buffer; any value is
+// correct, a large enough one avoids repeated rounds).
+std::atomic<double> g_prefix_rate{4e-4};
+
+// Candidate buffers of the prefix form, kept between calls: a call leases
+// one (concurrent calls on other streams lease others), grows it when its
+// estimate or its count needs more, and returns it. They hold device memory
+// until the process ends; growing one frees the old allocation.
+struct CandBuffer {
+  uint64_t* ptr = nullptr;
+  int64_t cap = 0;
+  int dev = -1;
+};
+std::mutex g_cand_mu;
+std::vector<CandBuffer> g_cand_pool;
+constexpr int64_t kCandMax = (int64_t)1 << 30;  // entries (8 GiB)
+
+class CandLease {
+ public:
+  CandLease() = default;
+  CandLease(const CandLease&) = delete;
+  CandLease& operator=(const CandLease&) = delete;
+  ~CandLease() {
+    if (!held_) return;
+    std::lock_guard<std::mutex> lock(g_cand_mu);
+    g_cand_pool.push_back(buf_);
+  }
+  // Takes a buffer of this device from the pool (or an empty one) and makes
+  // it hold at least `need` entries.
+  int acquire(int64_t need, hipStream_t s) {
+    int dev = 0;
+    FH_HIP(hipGetDevice(&dev));
+    {
+      std::lock_guard<std::mutex> lock(g_cand_mu);
+      for (size_t i = 0; i < g_cand_pool.size(); ++i) {
+        if (g_cand_pool[i].dev != dev) continue;
+        buf_ = g_cand_pool[i];
+        g_cand_pool.erase(g_cand_pool.begin() + (std::ptrdiff_t)i);
+        break;
+      }
+    }
+    buf_.dev = dev;
+    held_ = true;
+    return grow(need, s);
+  }
+  // At least `need` entries (at most kCandMax); earlier work on s that used
+  // the buffer is waited for before it is freed.
+  int grow(int64_t need, hipStream_t s) {
+    need = std::min(std::max<int64_t>(need, 1 << 20), kCandMax);
+    if (buf_.cap >= need) return FH_OK;
+    const int64_t cap = std::min(std::max(need, 2 * buf_.cap), kCandMax);
+    if (buf_.ptr) {
+      FH_HIP(hipStreamSynchronize(s));
+      FH_HIP(hipFree(buf_.ptr));
+      buf_.ptr = nullptr;
+      buf_.cap = 0;
+    }
+    hipError_t e = hipMalloc(&buf_.ptr, (size_t)cap * sizeof(uint64_t));
+    if (e != hipSuccess) {
+      buf_.ptr = nullptr;
+      set_error("fh_jw_levels: hipMalloc of the candidate buffer (%lld entries) failed: %s", (long long)cap,
+                hipGetErrorString(e));
+      return FH_ERR_HIP;
+    }
+    buf_.cap = cap;
+    return FH_OK;
+  }
+  uint64_t* ptr() const { return buf_.ptr; }
+  int64_t cap() const { return buf_.cap; }
+
+ private:
+  CandBuffer buf_;
+  bool held_ = false;
+};
 }  // namespace
 }  // namespace fh
 
 extern "C" int fh_jw_set_method(int method) {
   fh::clear_error();
-  FH_REQUIRE(method == FH_JW_METHOD_TILES || method == FH_JW_METHOD_BOUND, "fh_jw_set_method: bad method %d", method);
+  FH_REQUIRE(method == FH_JW_METHOD_TILES || method == FH_JW_METHOD_BOUND || method == FH_JW_METHOD_PREFIX,
+             "fh_jw_set_method: bad method %d", method);
   fh::g_jw_method.store(method);
   return FH_OK;
 }
@@ -121,19 +198,48 @@ struct Plan {
 
 static int lrange_of(int len) { return len <= 16 ? 0 : (len <= 32 ? 1 : 2); }
 
-// Launches the waves of length range r that lie in [u0, u1) against every tile kind.
+// Which (string-1 length range, tile kind) launches launch_kinds makes: all
+// with the per-lane kernel; only those the prefix form covers (string-1
+// values of up to 32 bytes against the packed and 32-bit kinds), with it;
+// or the rest, with the per-lane kernel.
+enum LaunchSet { kLaneAll, kPrefixOnly, kLaneRest };
+
+// Launches the waves of length range r that lie in [u0, u1) against the
+// tile kinds of `set` (LDS bytes: lds for the per-lane kernel, lds_prefix
+// for the prefix form).
 static int launch_kinds(const JwArgs* args, size_t lds, const Plan& pl, int r, int64_t u0, int64_t u1, hipStream_t s,
-                        int* launches) {
+                        int* launches, LaunchSet set = kLaneAll, size_t lds_prefix = 0) {
   const int64_t w0 = std::max(pl.wave_lo[r], u0), nw = std::min(pl.wave_hi[r], u1) - w0;
   if (nw <= 0) return FH_OK;
   for (int k = 0; k < kNumKinds; ++k) {
     const int64_t t0 = pl.tile_lo[k], nt = pl.tile_hi[k] - pl.tile_lo[k];
     if (nt <= 0) continue;
+    const bool covered = r < 2 && k != kW64;
+    if (set == kPrefixOnly) {
+      if (!covered) continue;
+      ++*launches;
+      if (r == 0) FH_TRY(launch_jw_prefix_r0(k, args, lds_prefix, w0, nw, t0, nt, s));
+      if (r == 1) FH_TRY(launch_jw_prefix_r1(k, args, lds_prefix, w0, nw, t0, nt, s));
+      continue;
+    }
+    if (set == kLaneRest && covered) continue;
     ++*launches;
     if (r == 0) FH_TRY(launch_jw_tiles_r0(k, args, lds, w0, nw, t0, nt, s));
     if (r == 1) FH_TRY(launch_jw_tiles_r1(k, args, lds, w0, nw, t0, nt, s));
     if (r == 2) FH_TRY(launch_jw_tiles_r2(k, args, lds, w0, nw, t0, nt, s));
   }
+  return FH_OK;
+}
+
+// Row codes of the sorted string-1 values [s_begin, s_end) (all of at most
+// 32 bytes) for the prefix form.
+static int launch_prefix_codes(const uint8_t* chars, const int64_t* off, const uint64_t* sorted, int64_t s_begin,
+                               int64_t s_end, const uint8_t* amap, int krows, uint32_t* codes, hipStream_t s) {
+  if (s_end <= s_begin) return FH_OK;
+  const int64_t blocks = std::min<int64_t>(ceil_div((s_end - s_begin) * 8, 256), 8192);
+  hipLaunchKernelGGL(k_prefix_codes, dim3((unsigned)blocks), dim3(256), 0, s, chars, off, sorted, s_begin, s_end, amap,
+                     krows, codes);
+  FH_HIP(hipGetLastError());
   return FH_OK;
 }
 
@@ -293,6 +399,9 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   // Bound-filtered form: string-2 tiles of kBTile strings of one length.
   const int method = g_jw_method.load();
   const bool bound = method == FH_JW_METHOD_BOUND;
+  // The prefix form needs 256-dword tiles (alphabets of up to about 55 symbols).
+  const bool prefix = method == FH_JW_METHOD_PREFIX && tw == kPrefixTileWidth;
+  const size_t lds_prefix = jw_prefix_lds_bytes(krows);
   std::vector<TileMeta> btiles;
   if (bound) {
     for (int c = 1; c <= kLongLen; ++c)
@@ -355,9 +464,10 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   const int64_t ntiles = (int64_t)pl.tiles.size();
   FH_TRY(scr.alloc(&d_table, (size_t)std::max<int64_t>(1, ntiles) * krows * tw));
   TileMeta* d_btiles = nullptr;
-  uint64_t* d_cand = nullptr;  // candidate pairs of one round of the bound pass
+  uint64_t* d_cand = nullptr;  // candidate pairs of one round (bound and prefix forms)
+  CandLease cand_lease;        // the prefix form's buffer, kept between calls
   unsigned long long* cand_count = nullptr;
-  const int64_t cand_cap = (int64_t)1 << 27;
+  int64_t cand_cap = (int64_t)1 << 27;
   int64_t* d_cls = nullptr;
   uint8_t *d_planes_a = nullptr, *d_planes_b = nullptr;
   const int64_t nbt = (int64_t)btiles.size();
@@ -370,22 +480,74 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     FH_TRY(jw_bound_build_planes(chars_a, offsets_a, ent_a, n_a, chars_b, offsets_b, ent_b, d_btiles, nbt, d_small, K,
                                  d_planes_a, d_planes_b, s));
   }
-  // The bound form's candidate pairs go to one buffer per round and are
-  // scored by a separate kernel; the bit-parallel form scores its survivors
-  // in the kernel. Both find a string-2 value's table column from its sorted
-  // index through the class starts and first tile of each class.
-  if (bound) {
-    FH_TRY(scr.alloc(&d_cand, (size_t)cand_cap));
+  // The bound and prefix forms' candidate pairs go to one buffer per round
+  // and are scored by a separate kernel; the per-lane bit-parallel form
+  // scores its survivors in the kernel. All find a string-2 value's table
+  // column from its sorted index through the class starts and first tile of
+  // each class.
+  //
+  // Rounds run over string-1 units (waves), each sized so that its
+  // candidates fit the buffer at the candidate rate seen so far; a round that
+  // overflows is split in two and redone (its comparison pass is repeated,
+  // nothing is lost). The prefix form sizes its buffer from the survivor rate
+  // of the previous calls.
+  int64_t unit_lo = 0, unit_hi = (int64_t)pl.waves.size();
+  std::vector<double> unit_pairs;  // prefix sums of kernel pairs per unit
+  double rate = 4e-3;              // candidates per kernel pair, updated after every round
+  const bool rounds = bound || prefix;
+  if (rounds) {
+    std::vector<double> per_len(kLongLen + 1, 0.0);
+    const int lbmax = prefix ? 32 : kLongLen;  // the prefix form covers string-2 values of up to 32 bytes
+    for (int la = 1; la <= kLongLen; ++la)
+      for (int lb = 1; lb <= lbmax; ++lb) {
+        if (!hb[lb] || mmin[la * (kLongLen + 1) + lb] > std::min(la, lb)) continue;
+        if (!two && lb > la) continue;  // deduplication: string 2 never longer than string 1
+        per_len[la] += (!two && lb == la) ? 0.5 * (double)hb[lb] : (double)hb[lb];
+      }
+    if (prefix) {
+      unit_lo = pl.wave_lo[0];
+      unit_hi = pl.wave_hi[1];
+    }
+    unit_pairs.assign((size_t)(unit_hi - unit_lo) + 1, 0.0);
+    for (int64_t u = unit_lo; u < unit_hi; ++u)
+      unit_pairs[u - unit_lo + 1] = unit_pairs[u - unit_lo] + (double)pl.waves[u].count * per_len[pl.waves[u].len];
+    if (prefix) {
+      rate = g_prefix_rate.load();
+      // The first round's budget (0.7 of the buffer at this rate) then covers all units.
+      FH_TRY(cand_lease.acquire((int64_t)std::min(1.5 * rate * unit_pairs.back(), (double)kCandMax), s));
+      d_cand = cand_lease.ptr();
+      cand_cap = cand_lease.cap();
+    } else {
+      FH_TRY(scr.alloc(&d_cand, (size_t)std::max<int64_t>(1, cand_cap)));
+    }
     FH_TRY(scr.alloc(&cand_count, 1));
   }
   {
-    std::vector<int64_t> cls(2 * kNumClasses, 0);
-    for (int c = 0; c < kNumClasses; ++c) cls[c] = start_b[c];
+    std::vector<int64_t> cls(3 * kNumClasses, 0);
+    for (int c = 0; c < kNumClasses; ++c) {
+      cls[c] = start_b[c];
+      cls[2 * kNumClasses + c] = start_a[c];
+    }
     for (int64_t t = (int64_t)pl.tiles.size() - 1; t >= 0; --t) cls[kNumClasses + pl.tiles[t].len] = t;
     FH_TRY(scr.alloc(&d_cls, cls.size()));
     FH_HIP(hipMemcpyAsync(d_cls, cls.data(), cls.size() * sizeof(int64_t), hipMemcpyHostToDevice, s));
   }
   FH_TRY(launch_build_table(chars_b, offsets_b, ent_b, d_tiles, ntiles, d_small, krows, tw, d_table, s));
+  uint32_t* d_codes = nullptr;
+  unsigned long long* d_skipped = nullptr;  // per string-1 length: steps skipped per tile pass (stats)
+  if (prefix) {
+    // Values of 1..32 bytes occupy sorted indices [start_a[1], start_a[33]).
+    FH_TRY(scr.alloc(&d_codes, (size_t)std::max<int64_t>(1, n_a) * 8));
+    FH_TRY(launch_prefix_codes(chars_a, offsets_a, ent_a, start_a[1], start_a[33], d_small, krows, d_codes, s));
+    if (stats && start_a[33] > start_a[1]) {
+      FH_TRY(scr.alloc(&d_skipped, 33));
+      FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(d_skipped), 0, 33, s));
+      const int64_t blocks = std::min<int64_t>(ceil_div(start_a[33] - start_a[1], 256), 4096);
+      hipLaunchKernelGGL(k_prefix_restarts, dim3((unsigned)blocks), dim3(256), 0, s, d_codes,
+                         (const int64_t*)(d_cls + 2 * kNumClasses), start_a[1], start_a[33], d_skipped);
+      FH_HIP(hipGetLastError());
+    }
+  }
   unsigned long long* counter = nullptr;  // [0] hits, [1] candidates
   FH_TRY(scr.alloc(&counter, 2));
   FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(counter), 0, 2, s));
@@ -421,18 +583,27 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   args.n_candidates = bound ? nullptr : counter + 1;  // survivors the bit-parallel pass scores
   args.cls_start = d_cls;
   args.cls_tile0 = d_cls ? d_cls + kNumClasses : nullptr;
+  args.cls_start_a = d_cls ? d_cls + 2 * kNumClasses : nullptr;
   // The bit-parallel form scores its (rare) survivors inside its VALU-bound
   // kernel; the bound form hands its candidates to the scoring kernel.
-  args.cand = bound ? d_cand : nullptr;
+  args.cand = rounds ? d_cand : nullptr;
   args.cand_cap = cand_cap;
   args.cand_count = cand_count;
   args.planes_a = d_planes_a;
   args.planes_b = d_planes_b;
+  args.codes_a = d_codes;
   if (bound) args.tiles = d_btiles;
 
+  // d_args: the arguments of the bound and prefix passes and of the scoring
+  // kernel; d_args_lane (no candidate buffer): the per-lane kernel, which
+  // scores its survivors itself.
   JwArgs* d_args = nullptr;
-  FH_TRY(scr.alloc(&d_args, 1));
+  FH_TRY(scr.alloc(&d_args, 2));
+  JwArgs* d_args_lane = d_args + 1;
   FH_HIP(hipMemcpyAsync(d_args, &args, sizeof(JwArgs), hipMemcpyHostToDevice, s));
+  JwArgs args_lane = args;
+  args_lane.cand = nullptr;
+  FH_HIP(hipMemcpyAsync(d_args_lane, &args_lane, sizeof(JwArgs), hipMemcpyHostToDevice, s));
   hipEvent_t ev0 = nullptr, ev1 = nullptr, evr[4] = {nullptr, nullptr, nullptr, nullptr};
   if (stats) {
     FH_HIP(hipEventCreate(&ev0));
@@ -443,9 +614,13 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   int launches = 0;
   int64_t n_candidates = 0;
   if (!bound) {
+    // The per-lane kernel: everything, or with the prefix form what it does
+    // not cover (string-1 values longer than 32 bytes, kW64 tiles).
     if (stats) FH_HIP(hipEventRecord(evr[0], s));
+    const LaunchSet set = prefix ? kLaneRest : kLaneAll;
     for (int r = 0; r < 3; ++r)
-      FH_TRY(launch_kinds(d_args, lds_bytes, pl, r, 0, (int64_t)pl.waves.size(), s, &launches));
+      FH_TRY(launch_kinds(d_args_lane, lds_bytes, pl, r, 0, (int64_t)pl.waves.size(), s, &launches, set,
+                          lds_prefix));
     if (stats) {
       FH_HIP(hipEventRecord(evr[1], s));
       FH_HIP(hipEventSynchronize(evr[1]));
@@ -453,32 +628,17 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       FH_HIP(hipEventElapsedTime(&ms, evr[0], evr[1]));
       stats->pass_ms += ms;
     }
-  } else {
-    // Rounds over string-1 units, each sized so that its candidates fit the
-    // buffer at the candidate rate seen so far; a round that overflows is
-    // split in two and redone (its comparison pass is repeated, nothing is
-    // lost).
-    const int64_t nunits = (int64_t)pl.waves.size();
-    std::vector<double> unit_pairs(nunits + 1, 0.0);  // prefix sums of kernel pairs per unit
-    {
-      std::vector<double> per_len(kLongLen + 1, 0.0);
-      for (int la = 1; la <= kLongLen; ++la)
-        for (int lb = 1; lb <= kLongLen; ++lb) {
-          if (!hb[lb] || mmin[la * (kLongLen + 1) + lb] > std::min(la, lb)) continue;
-          if (!two && lb > la) continue;  // deduplication: string 2 never longer than string 1
-          per_len[la] += (!two && lb == la) ? 0.5 * (double)hb[lb] : (double)hb[lb];
-        }
-      for (int64_t u = 0; u < nunits; ++u)
-        unit_pairs[u + 1] = unit_pairs[u] + (double)pl.waves[u].count * per_len[pl.waves[u].len];
-    }
-    double rate = 4e-3;  // candidates per kernel pair, updated after every round
-    std::vector<std::pair<int64_t, int64_t>> todo;
+  }
+  if (rounds) {
+    const int64_t nunits = unit_hi - unit_lo;
+    std::vector<std::pair<int64_t, int64_t>> todo;  // unit ranges relative to unit_lo
     auto next_round = [&](int64_t u0) {
       const double budget = 0.7 * (double)cand_cap / rate;
       int64_t u1 = u0 + 1;
       while (u1 < nunits && unit_pairs[u1 + 1] - unit_pairs[u0] <= budget) ++u1;
       return u1;
     };
+    double prefix_pairs = 0, prefix_got = 0;
     int64_t u0 = 0;
     while (u0 < nunits || !todo.empty()) {
       int64_t a0, a1;
@@ -493,8 +653,14 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       }
       FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(cand_count), 0, 1, s));
       if (stats) FH_HIP(hipEventRecord(evr[0], s));
-      FH_TRY(launch_jw_bound(mode, d_args, a0, a1 - a0, 0, nbt, s));
-      ++launches;
+      if (bound) {
+        FH_TRY(launch_jw_bound(mode, d_args, unit_lo + a0, a1 - a0, 0, nbt, s));
+        ++launches;
+      } else {
+        for (int r = 0; r < 2; ++r)
+          FH_TRY(launch_kinds(d_args, lds_bytes, pl, r, unit_lo + a0, unit_lo + a1, s, &launches, kPrefixOnly,
+                              lds_prefix));
+      }
       if (stats) FH_HIP(hipEventRecord(evr[1], s));
       uint64_t got = 0;
       FH_HIP(hipMemcpyAsync(&got, cand_count, sizeof(got), hipMemcpyDeviceToHost, s));
@@ -507,16 +673,46 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       const double pairs = unit_pairs[a1] - unit_pairs[a0];
       if (pairs > 0) rate = std::max(1e-6, 1.25 * (double)got / pairs);
       if ((int64_t)got > cand_cap) {
-        FH_REQUIRE(a1 - a0 > 1, "fh_jw_levels: %llu candidates in one string-1 unit exceed the buffer",
-                   (unsigned long long)got);
+        if (prefix || a1 - a0 == 1) {
+          // A larger buffer and the same round again: the prefix form's
+          // buffer is kept for later calls; a single unit cannot be split.
+          if (prefix) {
+            FH_TRY(cand_lease.grow((int64_t)((double)got * 1.1) + 1, s));
+            FH_REQUIRE(cand_lease.cap() >= (int64_t)got || a1 - a0 > 1,
+                       "fh_jw_levels: %llu candidates in one string-1 unit exceed the buffer",
+                       (unsigned long long)got);
+            if (cand_lease.cap() < (int64_t)got) {
+              const int64_t mid = a0 + (a1 - a0) / 2;
+              todo.push_back({mid, a1});
+              todo.push_back({a0, mid});
+            } else {
+              todo.push_back({a0, a1});
+            }
+            d_cand = cand_lease.ptr();
+            cand_cap = cand_lease.cap();
+          } else {
+            cand_cap = (int64_t)got;
+            FH_TRY(scr.alloc(&d_cand, (size_t)cand_cap));
+            todo.push_back({a0, a1});
+          }
+          args.cand = d_cand;
+          args.cand_cap = cand_cap;
+          FH_HIP(hipMemcpyAsync(d_args, &args, sizeof(JwArgs), hipMemcpyHostToDevice, s));
+          continue;
+        }
         const int64_t mid = a0 + (a1 - a0) / 2;
         todo.push_back({mid, a1});
         todo.push_back({a0, mid});
         continue;
       }
+      prefix_pairs += pairs;
+      prefix_got += (double)got;
       n_candidates += (int64_t)got;
       if (stats) FH_HIP(hipEventRecord(evr[2], s));
-      FH_TRY(launch_eval_candidates(d_args, (int64_t)got, s));
+      if (bound)
+        FH_TRY(launch_eval_candidates(d_args, (int64_t)got, s));
+      else
+        FH_TRY(launch_eval_prefix(d_args, (int64_t)got, s));
       ++launches;
       if (stats) {
         FH_HIP(hipEventRecord(evr[3], s));
@@ -526,6 +722,8 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
         stats->eval_ms += ms;
       }
     }
+    if (prefix && prefix_pairs > 0)
+      g_prefix_rate.store(std::max({1e-6, prefix_got / prefix_pairs, 0.95 * g_prefix_rate.load()}));
   }
 
   // Generic pairs: special strings (longer than 64 bytes, and empty strings
@@ -568,8 +766,29 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   }
   uint64_t h_count[2] = {0, 0};
   FH_HIP(hipMemcpyAsync(h_count, counter, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  std::vector<unsigned long long> h_skipped(33, 0);
+  if (d_skipped)
+    FH_HIP(hipMemcpyAsync(h_skipped.data(), d_skipped, 33 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   FH_HIP(hipStreamSynchronize(s));
-  if (stats) stats->pairs_candidates = bound ? n_candidates : (int64_t)h_count[1];
+  if (stats && d_skipped) {
+    // Each skipped step saves three ops on every string-2 dword the value
+    // meets: 3 / L lane-ops per pair of a kind with L strings per dword.
+    double skipped_ops = 0;
+    for (int la = 1; la <= 32; ++la) {
+      if (!h_skipped[la]) continue;
+      double per_step = 0;
+      for (int lb = 1; lb <= 32; ++lb) {
+        if (!hb[lb] || mmin[la * (kLongLen + 1) + lb] > std::min(la, lb)) continue;
+        if (!two && lb > la) continue;
+        const int k = kind_of_len(lb);
+        const double nb = (!two && lb == la) ? 0.5 * (double)hb[lb] : (double)hb[lb];
+        per_step += nb * 3.0 / (packed_kind(k) ? kind_lanes(k) : 1);
+      }
+      skipped_ops += (double)h_skipped[la] * per_step;
+    }
+    stats->prefix_skipped_ops = skipped_ops;
+  }
+  if (stats) stats->pairs_candidates = bound ? n_candidates : n_candidates + (int64_t)h_count[1];
   if (stats) {
     float ms = 0;
     FH_HIP(hipEventElapsedTime(&ms, ev0, ev1));

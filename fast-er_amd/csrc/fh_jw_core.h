@@ -185,6 +185,11 @@ struct JwArgs {
   uint64_t* cand;                    // candidate buffer of the bound pass
   int64_t cand_cap;
   unsigned long long* cand_count;
+  // Prefix form (fh_jw_prefix.h): row codes of every string-1 value of at
+  // most 32 bytes, [sorted index][8 dwords], four codes per dword, and the
+  // first string-1 sorted index of each length class.
+  const uint32_t* codes_a;
+  const int64_t* cls_start_a;  // [kNumClasses]
 };
 
 constexpr unsigned long long kEmptyKey = ~0ull;

@@ -13,7 +13,9 @@ import threading
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfaster_hip.so")
+# FASTER_HIP_LIB names another build of the same library (for timing
+# compile-time variants side by side); the default is the in-tree build.
+LIB_PATH = os.environ.get("FASTER_HIP_LIB") or os.path.join(_HERE, "libfaster_hip.so")
 
 FH_MODE_LINK = 0
 FH_MODE_DEDUP = 1
@@ -47,6 +49,7 @@ class JwStats(ctypes.Structure):
         ("pass_ms", ctypes.c_double),
         ("eval_ms", ctypes.c_double),
         ("frozen_model_ops", ctypes.c_double),
+        ("prefix_skipped_ops", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -169,7 +172,8 @@ def device() -> torch.device:
 
 FH_JW_METHOD_TILES = 0
 FH_JW_METHOD_BOUND = 1
-_METHODS = {"tiles": FH_JW_METHOD_TILES, "bound": FH_JW_METHOD_BOUND}
+FH_JW_METHOD_PREFIX = 2
+_METHODS = {"tiles": FH_JW_METHOD_TILES, "bound": FH_JW_METHOD_BOUND, "prefix": FH_JW_METHOD_PREFIX}
 
 
 def set_jw_method(name: str) -> None:

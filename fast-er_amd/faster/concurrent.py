@@ -6,10 +6,12 @@ GPU idle between launches when the variables run one after another. Here
 every variable runs from its own host thread on its own stream (default
 priority), so one variable's short kernels and readbacks overlap another's
 comparison kernel. The library calls release the GIL (ctypes), and torch's
-current stream is per thread.
+current stream is per thread. FASTER_SERIAL_VARS=1 runs the variables one
+after another on the caller's stream (for per-kernel timing).
 """
 from __future__ import annotations
 
+import os
 from concurrent.futures import ThreadPoolExecutor
 
 import torch
@@ -17,6 +19,9 @@ import torch
 from . import _lib
 
 _streams = {}
+# True: run() calls the tasks one after another on the caller's stream (the
+# per-kernel timings of the work accounting then do not overlap).
+SERIAL = False
 
 
 def _side_streams(dev, k):
@@ -31,7 +36,7 @@ def run(tasks):
     first element is a tensor) concurrently, each on a side stream ordered
     after the caller's current stream; returns their results in order once
     all of their device work has finished."""
-    if len(tasks) <= 1:
+    if len(tasks) <= 1 or SERIAL or os.environ.get("FASTER_SERIAL_VARS") == "1":
         return [t() for t in tasks]
     dev = _lib.device()
     main = torch.cuda.current_stream(dev)

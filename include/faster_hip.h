@@ -80,22 +80,31 @@ typedef struct {
   double eval_ms;          /* device time of the candidate scoring kernels (HIP events) */
   double frozen_model_ops; /* sum over the evaluated pairs of 8 * len1 + 24: a fixed work model of a
                               non-packed recurrence (8 ops per string-1 character + 24 per pair) */
+  double prefix_skipped_ops; /* FH_JW_METHOD_PREFIX: lane-ops of alg_lane_ops that the reuse of shared
+                                prefixes skips (recurrence steps not recomputed); the executed
+                                algorithmic work is alg_lane_ops - prefix_skipped_ops */
 } fh_jw_stats;
 
 /*
- * Comparison algorithm of fh_jw_levels (process-wide; both give identical
+ * Comparison algorithm of fh_jw_levels (process-wide; all give identical
  * results):
- *   FH_JW_METHOD_TILES (default): the bit-parallel greedy match of every pair
- *     (VALU).
+ *   FH_JW_METHOD_TILES: the bit-parallel greedy match of every pair (VALU),
+ *     one string-1 value per lane.
  *   FH_JW_METHOD_BOUND: an int8 MFMA pass bounds every pair's match count by
  *     its symbol-count overlap; pairs whose bound reaches the pruning
  *     threshold are scored exactly.
+ *   FH_JW_METHOD_PREFIX (default): the same bit-parallel match with the
+ *     string-1 values walked one after another by each wave, whose lanes
+ *     cover the string-2 tile; the match state of a prefix shared with the
+ *     previous (sorted) value is reused. Values longer than 32 bytes and
+ *     string-2 values longer than 32 bytes use the TILES kernel.
  * No counterpart in the reference (one CUDA kernel scores every pair,
- * faster/comparison.py:133-186); the switch exists to time and check the two
+ * faster/comparison.py:133-186); the switch exists to time and check the
  * forms against each other.
  */
 #define FH_JW_METHOD_TILES 0
 #define FH_JW_METHOD_BOUND 1
+#define FH_JW_METHOD_PREFIX 2
 int fh_jw_set_method(int method);
 int fh_jw_get_method(void);
 

@@ -514,11 +514,15 @@ def test_blocked_linkage_matches_per_block_loop(oracle):
     np.testing.assert_array_equal(bl.Block_Counts, ref)
 
 
-@pytest.mark.parametrize("mode", ["link", "dedup"])
-def test_comparison_methods_agree(mode):
-    """The MFMA bound-filtered form and the bit-parallel form return the same
-    level keys on synthetic data with every length class (1..64 bytes, every
-    mask kind) and many near-duplicates."""
+@pytest.mark.parametrize("mode,order", [("link", "random"), ("dedup", "random"), ("link", "sorted"),
+                                        ("dedup", "sorted")])
+def test_comparison_methods_agree(mode, order):
+    """The MFMA bound-filtered form, the per-lane bit-parallel form and the
+    prefix form return the same level keys on synthetic data with every length
+    class (1..64 bytes, every mask kind) and many near-duplicates. "sorted"
+    passes the values in byte order over a small alphabet, so consecutive
+    values share long prefixes and the prefix form reuses states at every
+    depth; "random" leaves them unordered."""
     from faster import _lib, _ops
     from faster.synthetic import random_strings
 
@@ -528,10 +532,15 @@ def test_comparison_methods_agree(mode):
     for k in range(0, 2500, 3):
         s = A[(k * 7) % 3000]
         B[k] = s[:-1] + "x" if len(s) > 1 else s
+    if order == "sorted":
+        # Few symbols, so that many values of one length share prefixes.
+        tr = str.maketrans("abcdefghijklmnopqrstuvwxyz", "abcabcabcabcabcabcabcabcab")
+        A = sorted({s.translate(tr) for s in A}, key=lambda s: s.encode())
+        B = sorted({s.translate(tr) for s in B} | set(A[::5]), key=lambda s: s.encode())
     dev = _lib.device()
     out = {}
     default = _lib.jw_method()
-    for method in ("tiles", "bound"):
+    for method in ("tiles", "bound", "prefix"):
         _lib.set_jw_method(method)
         try:
             if mode == "link":
@@ -549,6 +558,7 @@ def test_comparison_methods_agree(mode):
         out[method] = np.sort(hits.cpu().numpy())
     assert out["tiles"].size > 500
     np.testing.assert_array_equal(out["bound"], out["tiles"])
+    np.testing.assert_array_equal(out["prefix"], out["tiles"])
 
 
 def _pack(strings, dev):
