@@ -128,6 +128,7 @@ __global__ void __launch_bounds__(256) k_jw_generic(JwArgs a, int64_t sp0, int64
     const int lb = (int)(a.off_b[eb + 1] - ob);
     const bool sa_special = la > kLongLen || (empty_special && la == 0);
     const bool sb_special = lb > kLongLen || (empty_special && lb == 0);
+    if (la > kMaxGenericLen || lb > kMaxGenericLen) continue;  // k_jw_huge
     if (a.mode == FH_MODE_DEDUP) {
       // Special x against every y != x; a pair of two specials once (x > y).
       if (ea == eb) continue;
@@ -147,6 +148,58 @@ __global__ void __launch_bounds__(256) k_jw_generic(JwArgs a, int64_t sp0, int64
       yy = ea;
     } else {
       s = jw_exact<kMaxGenericLen / 64>(sa, la, sb, lb, a.p);
+    }
+    const int lvl = level_of(s, a.lo, a.hi);
+    if (lvl > 0) {
+      const unsigned long long pos = atomicAdd(a.counter, 1ull);
+      if ((int64_t)pos < a.cap) a.hits[pos] = ((uint64_t)xx * (uint64_t)a.n_b + yy) * 4ull + (uint64_t)lvl;
+    }
+  }
+}
+
+// The pairs of k_jw_generic's row sets that have a value over
+// kMaxGenericLen bytes, scored with their flags in global scratch
+// (jw_exact_mem): a grid-stride loop over the (row, other) pairs, every
+// thread owning words [k * nthreads + tid] of `scratch`.
+__global__ void __launch_bounds__(256) k_jw_huge(JwArgs a, int64_t sp0, int64_t nsp, int64_t n_other, int swap,
+                                                 uint64_t* __restrict__ scratch) {
+  const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t* flags = scratch + tid;
+  const int64_t total = nsp * n_other;
+  for (int64_t q = tid; q < total; q += nthr) {
+    const int64_t r = q / n_other, x = q - r * n_other;
+    uint64_t ea, eb;
+    if (!swap) {
+      ea = a.sorted_a[sp0 + r];
+      eb = (uint64_t)x;
+    } else {
+      eb = a.sorted_b[sp0 + r];
+      ea = (uint64_t)x;
+    }
+    const int64_t oa = a.off_a[ea];
+    const int64_t la = a.off_a[ea + 1] - oa;
+    const int64_t ob = a.off_b[eb];
+    const int64_t lb = a.off_b[eb + 1] - ob;
+    if (la <= kMaxGenericLen && lb <= kMaxGenericLen) continue;
+    // The same pair ownership as k_jw_generic (empty strings never reach
+    // here: a pair with a value this long has a non-empty member).
+    if (a.mode == FH_MODE_DEDUP) {
+      if (ea == eb) continue;
+      if (lb > kLongLen && ea < eb) continue;
+    } else if (swap) {
+      if (la > kLongLen) continue;
+    }
+    const uint8_t* sa = a.chars_a + oa;
+    const uint8_t* sb = a.chars_b + ob;
+    uint64_t xx = ea, yy = eb;
+    float s;
+    if (a.mode == FH_MODE_DEDUP && !lex_greater(sa, (int)la, sb, (int)lb)) {
+      s = jw_exact_mem(sb, (int)lb, sa, (int)la, a.p, flags, nthr);
+      xx = eb;
+      yy = ea;
+    } else {
+      s = jw_exact_mem(sa, (int)la, sb, (int)lb, a.p, flags, nthr);
     }
     const int lvl = level_of(s, a.lo, a.hi);
     if (lvl > 0) {
@@ -313,13 +366,8 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   const uint32_t* pa = hmeta.data() + 2 * kNumClasses;
   const uint32_t* pb = two ? pa + 8 : pa;
 
-  {
-    const uint32_t ml = std::max(hmeta[2 * kNumClasses + 16], two ? hmeta[2 * kNumClasses + 17] : 0u);
-    if (ml > (uint32_t)kMaxGenericLen) {
-      set_error("fh_jw_levels: strings of %u bytes exceed the supported maximum of %d", ml, kMaxGenericLen);
-      return FH_ERR_UNSUPPORTED;
-    }
-  }
+  // Longest value (values over kMaxGenericLen bytes are scored by k_jw_huge).
+  const uint32_t max_len = std::max(hmeta[2 * kNumClasses + 16], two ? hmeta[2 * kNumClasses + 17] : 0u);
   const float lo = lower_thr, hi = upper_thr;
   const bool empty_special = host_reaches(0.0f, lo, hi);
 
@@ -760,6 +808,25 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       gpairs += (double)nsb_long * n_a;
     }
     if (stats) stats->pairs_generic = (int64_t)gpairs;
+    // Values over kMaxGenericLen bytes: their flags in scratch, about 1 GiB
+    // at most (fewer threads for longer values), 256 threads at least.
+    if (max_len > (uint32_t)kMaxGenericLen) {
+      const int64_t words = 2 * (((int64_t)max_len + 63) / 64);
+      int64_t threads = std::min<int64_t>(65536, ((int64_t)1 << 27) / words);
+      threads = std::max<int64_t>(256, threads / 256 * 256);
+      uint64_t* scratch = nullptr;
+      FH_TRY(scr.alloc(&scratch, (size_t)(threads * words)));
+      auto huge = [&](int64_t sp0, int64_t nsp, int64_t n_other, int swap) -> int {
+        if (nsp <= 0 || n_other <= 0) return FH_OK;
+        hipLaunchKernelGGL(k_jw_huge, dim3((unsigned)(threads / 256)), dim3(256), 0, s, args, sp0, nsp, n_other, swap,
+                           scratch);
+        FH_HIP(hipGetLastError());
+        ++launches;
+        return FH_OK;
+      };
+      FH_TRY(huge(start_a[kClassLong], nsa_long, n_b, 0));
+      if (two) FH_TRY(huge(start_b[kClassLong], nsb_long, n_a, 1));
+    }
   }
   if (stats) {
     FH_HIP(hipEventRecord(ev1, s));

@@ -52,6 +52,46 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(F
   }
 }
 
+// Pairs of one block with a value over kLongLen bytes, which the
+// bit-parallel passes do not cover: sorted side-A indices [i0, i0 + ni)
+// against sorted side-B indices [j0, j0 + nj) of variable `var` (A's long
+// values against all of the block's B values, and A's other values against
+// B's long ones).
+struct LongItem {
+  int32_t var;
+  int32_t ni;
+  int64_t i0;
+  int64_t j0;
+  int64_t nj;
+};
+
+// Scores the long-value pairs of every item (the reference's function, with
+// its flags in registers up to kMaxGenericLen bytes and in the scratch
+// words [k * nthreads + tid] beyond) and adds their levels to the pair table.
+// Empty values pair at level 0 (the blocked plan requires thresholds above
+// 0).
+__global__ void __launch_bounds__(256) k_blk_long(const JwArgs* __restrict__ vars, const LongItem* __restrict__ items,
+                                                  int64_t n_items, uint64_t* __restrict__ scratch) {
+  const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+  uint64_t* flags = scratch + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const LongItem li = items[it];
+    const JwArgs& a = vars[li.var];
+    const int64_t total = (int64_t)li.ni * li.nj;
+    for (int64_t q = threadIdx.x; q < total; q += blockDim.x) {
+      const int64_t r = q / li.nj;
+      const uint64_t ea = a.sorted_a[li.i0 + r], eb = a.sorted_b[li.j0 + (q - r * li.nj)];
+      const int64_t oa = a.off_a[ea], ob = a.off_b[eb];
+      const int64_t la = a.off_a[ea + 1] - oa, lb = a.off_b[eb + 1] - ob;
+      if (la == 0 || lb == 0) continue;
+      const float sc = (la <= kMaxGenericLen && lb <= kMaxGenericLen)
+                           ? jw_exact<kMaxGenericLen / 64>(a.chars_a + oa, (int)la, a.chars_b + ob, (int)lb, a.p)
+                           : jw_exact_mem(a.chars_a + oa, (int)la, a.chars_b + ob, (int)lb, a.p, flags, nthr);
+      emit_level(a, ea, eb, level_of(sc, a.lo, a.hi));
+    }
+  }
+}
+
 struct ExactRowIds {
   const int64_t* a[kMaxBlkExact];
   const int64_t* b[kMaxBlkExact];
@@ -143,6 +183,11 @@ struct BlockedPlan {
   std::vector<int64_t> items_blk_lo[2];
   std::vector<std::vector<int64_t>> tile_blk_lo;
   unsigned int* heads_blk = nullptr;
+  // Long-value pairs (k_blk_long): items, a persistent grid and its scratch.
+  LongItem* long_dev = nullptr;
+  int64_t n_long = 0;
+  int long_grid = 0;
+  uint64_t* long_scratch = nullptr;
   unsigned long long* hkeys = nullptr;
   uint32_t* hcodes = nullptr;
   uint64_t hslots = 0;
@@ -201,7 +246,16 @@ struct BlockedPlan {
                          items_dev[1], n_items[1], heads + 1);
       FH_HIP(hipGetLastError());
     }
+    FH_TRY(enqueue_long());
     return enqueue_hist();
+  }
+
+  int enqueue_long() {
+    if (n_long == 0) return FH_OK;
+    hipLaunchKernelGGL(k_blk_long, dim3((unsigned)long_grid), dim3(256), 0, stream, args_dev, long_dev, n_long,
+                       long_scratch);
+    FH_HIP(hipGetLastError());
+    return FH_OK;
   }
 
   int enqueue_hist() {
@@ -243,6 +297,8 @@ struct BlockedPlan {
         FH_HIP(hipGetLastError());
       }
     }
+    // The long-value pairs of every block in one launch.
+    FH_TRY(enqueue_long());
     return enqueue_hist();
   }
 };
@@ -316,6 +372,8 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
   std::vector<uint8_t> mmin((kLongLen + 1) * (kLongLen + 1), 255);
   std::vector<bool> mm_done((kLongLen + 1) * (kLongLen + 1), false);
   std::vector<WorkItem> items[2];
+  std::vector<LongItem> long_items;
+  uint32_t max_len = 0;
   std::vector<int32_t> item_keys[2];  // tile kind * 256 + string-1 length of each item
   std::vector<int32_t> item_block[2];
   int max_krows = 1;
@@ -354,10 +412,7 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
     FH_HIP(hipMemcpyAsync(hb.data(), hist_b, nk * 4, hipMemcpyDeviceToHost, s));
     FH_HIP(hipMemcpyAsync(hm.data(), meta, 32 * 4, hipMemcpyDeviceToHost, s));
     FH_HIP(hipStreamSynchronize(s));
-    if (hm[16] > 0 || hm[17] > 0) {
-      set_error("fh_blocked_plan_create: strings longer than %d bytes need the plain path", kLongLen);
-      return FH_ERR_UNSUPPORTED;
-    }
+    max_len = std::max({max_len, hm[16], hm[17]});
     // Alphabet of this variable.
     std::vector<uint8_t> amap(256);
     int K = 0;
@@ -378,6 +433,31 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
     for (int32_t b = 0; b < n_blocks; ++b) {
       wave_lo[b] = (int64_t)waves.size();
       tile_lo[b] = (int64_t)tiles.size();
+      {
+        // Pairs with a long value (class kClassLong, the last of the block):
+        // A's long values x all B values, A's other values x B's long ones.
+        int64_t na_blk = 0, nb_blk = 0;
+        for (int c = 0; c < kNumClasses; ++c) {
+          na_blk += ha[(int64_t)b * kNumClasses + c];
+          nb_blk += hb[(int64_t)b * kNumClasses + c];
+        }
+        const int64_t na_long = ha[(int64_t)b * kNumClasses + kClassLong];
+        const int64_t nb_long = hb[(int64_t)b * kNumClasses + kClassLong];
+        auto add_long = [&](int64_t i0, int64_t ni, int64_t j0, int64_t nj) {
+          if (ni <= 0 || nj <= 0) return;
+          for (int64_t k = 0; k < ni; k += 64) {
+            LongItem li;
+            li.var = v;
+            li.ni = (int32_t)std::min<int64_t>(64, ni - k);
+            li.i0 = i0 + k;
+            li.j0 = j0;
+            li.nj = nj;
+            long_items.push_back(li);
+          }
+        };
+        add_long(sa + na_blk - na_long, na_long, sb, nb_blk);
+        add_long(sa, na_blk - na_long, sb + nb_blk - nb_long, nb_long);
+      }
       for (int c = 0; c < kNumClasses; ++c) {
         const int64_t ca = ha[(int64_t)b * kNumClasses + c], cb = hb[(int64_t)b * kNumClasses + c];
         if (c == 33) wave_short_end[b] = (int64_t)waves.size();
@@ -566,6 +646,22 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
                             hipMemcpyHostToDevice, s));
   }
   FH_TRY(pl->alloc(&pl->heads_blk, 2 * (size_t)n_blocks));
+  // Long-value pairs: a persistent grid of up to 256 workgroups, fewer when
+  // values over kMaxGenericLen bytes need more than about 1 GiB of flags.
+  pl->n_long = (int64_t)long_items.size();
+  if (pl->n_long > 0) {
+    FH_TRY(pl->alloc(&pl->long_dev, long_items.size()));
+    FH_HIP(hipMemcpyAsync(pl->long_dev, long_items.data(), long_items.size() * sizeof(LongItem), hipMemcpyHostToDevice,
+                          s));
+    int64_t wgs = std::min<int64_t>(256, pl->n_long);
+    int64_t words = 1;
+    if (max_len > (uint32_t)kMaxGenericLen) {
+      words = 2 * (((int64_t)max_len + 63) / 64);
+      wgs = std::max<int64_t>(1, std::min<int64_t>(wgs, ((int64_t)1 << 27) / (words * 256)));
+    }
+    pl->long_grid = (int)wgs;
+    FH_TRY(pl->alloc(&pl->long_scratch, (size_t)(wgs * 256 * words)));
+  }
   const size_t qbytes = (size_t)kWaves * kQCap * sizeof(uint64_t);
   pl->lds_bytes = ((size_t)max_krows * (pl->tw + 1) + 1) * 4 + qbytes;
   FH_REQUIRE(pl->lds_bytes <= 64 * 1024, "fh_blocked_plan_create: alphabet too large for tile width %d", pl->tw);

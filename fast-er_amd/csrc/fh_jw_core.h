@@ -72,7 +72,7 @@ static_assert(kU == 4 || kU == 8, "kU must be 4 or 8");
 #endif
 constexpr int kQCap = 128;            // survivor queue entries per wave
 constexpr int kLongLen = 64;          // longest string on the bit-parallel path
-constexpr int kMaxGenericLen = 1024;  // longest string supported at all
+constexpr int kMaxGenericLen = 1024;  // longest string scored with its flags in registers
 constexpr int kClassLong = kLongLen + 1;
 constexpr int kNumClasses = kLongLen + 2;  // 0 = empty, 1..64, 65 = long
 constexpr int kBTile = 256;                // string-2 strings per tile of the bound-filtered form
@@ -306,6 +306,60 @@ __device__ float jw_exact(const uint8_t* __restrict__ s1, int l1, const uint8_t*
   for (int k = 0; k < m; ++k) {
     while (f == 0) f = flag1[++wi];
     while (u == 0) u = used2[++wj];
+    const int i = wi * 64 + __builtin_ctzll(f);
+    const int j = wj * 64 + __builtin_ctzll(u);
+    if (s1[i] != s2[j]) ++tt;
+    f &= f - 1;
+    u &= u - 1;
+  }
+  int pre = 0;
+  const int lim = min(min(l1, l2), 4);
+  for (int i = 0; i < lim; ++i) {
+    if (s1[i] == s2[i])
+      ++pre;
+    else
+      break;
+  }
+  return jw_formula(l1, l2, m, tt, pre, p);
+}
+
+// The same score for values of any length (the reference sizes its work
+// arrays from the offsets, faster/comparison.py:304-307): the used
+// positions of string 2 and the matched positions of string 1 are bit flags
+// in global memory, word k of this thread's flags at flags[k * stride]
+// (string 2's ceil(l2 / 64) words first, then string 1's), so that threads
+// sharing a scratch buffer interleave their words.
+__device__ __noinline__ float jw_exact_mem(const uint8_t* __restrict__ s1, int l1, const uint8_t* __restrict__ s2,
+                                           int l2, float p, uint64_t* __restrict__ flags, int64_t stride) {
+  if (l1 == 0 || l2 == 0) return 0.0f;
+  const int w = max(l1, l2) / 2 - 1;
+  const int w2 = (l2 + 63) >> 6, w1 = (l1 + 63) >> 6;
+  uint64_t* used2 = flags;
+  uint64_t* flag1 = flags + (int64_t)w2 * stride;
+  for (int k = 0; k < w2; ++k) used2[(int64_t)k * stride] = 0;
+  for (int k = 0; k < w1; ++k) flag1[(int64_t)k * stride] = 0;
+  int m = 0;
+  for (int i = 0; i < l1; ++i) {
+    const uint8_t c = s1[i];
+    const int jlo = max(0, i - w);
+    const int jhi = min(l2, i + w + 1);
+    for (int j = jlo; j < jhi; ++j) {
+      if (s2[j] != c) continue;
+      uint64_t* u = used2 + (int64_t)(j >> 6) * stride;
+      if ((*u >> (j & 63)) & 1ull) continue;
+      *u |= 1ull << (j & 63);
+      flag1[(int64_t)(i >> 6) * stride] |= 1ull << (i & 63);
+      ++m;
+      break;
+    }
+  }
+  if (m == 0) return 0.0f;
+  int tt = 0;
+  int wi = 0, wj = 0;
+  uint64_t f = flag1[0], u = used2[0];
+  for (int k = 0; k < m; ++k) {
+    while (f == 0) f = flag1[(int64_t)(++wi) * stride];
+    while (u == 0) u = used2[(int64_t)(++wj) * stride];
     const int i = wi * 64 + __builtin_ctzll(f);
     const int j = wj * 64 + __builtin_ctzll(u);
     if (s1[i] != s2[j]) ++tt;

@@ -67,7 +67,7 @@ typedef struct {
   int64_t pairs_logical;   /* value pairs the call covers */
   int64_t pairs_kernel;    /* pairs evaluated by the comparison kernel (not pruned by length) */
   int64_t pairs_pruned;    /* pairs skipped because their lengths cannot reach level 1 */
-  int64_t pairs_generic;   /* pairs evaluated by the generic kernel (empty or > 64 bytes) */
+  int64_t pairs_generic;   /* pairs evaluated by the generic kernels (empty or > 64 bytes; any length) */
   double alg_lane_ops;     /* algorithmic VALU lane-ops of the kernel pairs */
   int32_t alphabet;        /* byte values present on both sides */
   int32_t launches;        /* kernel launches of the comparison stage */
@@ -159,8 +159,10 @@ int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, int64_t n_a,
  * flag (the pair table was too small: rerun with more hash_slots) and the
  * device time of the run in ms. Patterns whose fuzzy digits are all 0 are not
  * counted here (their sizes follow from value-group counts per block).
- * Strings must be 1..64 bytes (empty strings are allowed when a score of 0
- * is below both thresholds).
+ * Strings may have any length: values of 1..64 bytes go through the
+ * bit-parallel passes, pairs with a longer value through a scoring kernel of
+ * their own (flags in registers up to 1,024 bytes, in plan-allocated scratch
+ * beyond). Empty strings pair at level 0; the thresholds must be above 0.
  */
 typedef struct {
   int32_t tile_width;     /* string-2 tile width in dwords: 32, 64, 128 or 256 */

@@ -109,6 +109,35 @@ def test_all_pairs_levels(oracle, seed, alpha, lo_len, hi_len):
     _eq_lists(got, ref, f"seed {seed}")
 
 
+@pytest.mark.parametrize("lo,hi", [(0.88, 0.94), (0.0, 0.5)])
+def test_values_of_any_length(oracle, lo, hi):
+    """Values over 1,024 bytes (scored with their flags in global scratch)
+    beside short, long (65-1,024 bytes) and empty ones, in linkage and
+    deduplication: near copies of the long values reach both levels, and
+    thresholds at 0 make every pair a hit, so each pair with a long member is
+    emitted exactly once."""
+    from faster.comparison import jaro_winkler_gpu
+    from faster.deduplication import jaro_winkler_dedup_gpu
+
+    rng = np.random.default_rng(13)
+
+    def near(s, k):
+        b = list(s)
+        for i in rng.integers(0, len(b), k):
+            b[int(i)] = "abcd"[int(rng.integers(0, 4))]
+        return "".join(b)
+
+    huge = _words(rng, 6, "abcd", 1025, 2600)
+    A = _words(rng, 30, "abcd", 0, 12) + huge + _words(rng, 3, "abcd", 65, 300) + [""]
+    B = _words(rng, 30, "abcd", 0, 12) + [near(h, 3) for h in huge[:4]] + [huge[4] + "ab", huge[5][:-1]] + \
+        _words(rng, 2, "abcd", 900, 1100) + ["", huge[0][:40]]
+    _eq_lists(jaro_winkler_gpu(A, B, 5, 0.1, lo, hi), oracle.jaro_winkler_all_pairs(A, B, 5, 0.1, lo, hi),
+              f"link any length {lo},{hi}")
+    S = A[25:] + B[28:]
+    _eq_lists(jaro_winkler_dedup_gpu(S, 0.1, lo, hi), oracle.jaro_winkler_dedup(S, 0.1, lo, hi),
+              f"dedup any length {lo},{hi}")
+
+
 @pytest.mark.parametrize("lo,hi", [(0.88, 0.94), (0.7, 0.8), (0.94, 0.88), (0.0, 0.5), (0.99, 1.0)])
 def test_threshold_edge_cases(oracle, lo, hi):
     """Thresholds at and below 0 (every pair is a hit), inverted, and 1.0."""
@@ -512,6 +541,45 @@ def test_blocked_linkage_matches_per_block_loop(oracle):
     assert res
     bl.fit()
     np.testing.assert_array_equal(bl.Block_Counts, ref)
+
+
+def test_blocked_linkage_with_long_values(oracle):
+    """Blocks holding values over 64 bytes (scored outside the bit-parallel
+    passes) and over 1,024 bytes (flags in scratch), with near copies across
+    the sides of one block: the batched launch, the per-block launch loop and
+    the per-block recipe give the same Counts, and one block equals the
+    oracle."""
+    from faster.blocking import BlockedLinkage, per_block_loop
+    from faster.synthetic import blocked_tables
+
+    dfa, dfb, ba, bb, sizes = blocked_tables(2400, n_blocks=10, seed=5, kmin=20, kmax=800)
+    rng = np.random.default_rng(6)
+    vf = ["first_name", "last_name", "street"]
+    b0 = int(np.argmax(sizes))
+    ra, rb = np.nonzero(ba == b0)[0], np.nonzero(bb == b0)[0]
+    rb2 = np.nonzero(bb == (b0 + 1) % len(sizes))[0]
+    for v, (lo, hi) in zip(vf, [(65, 200), (1025, 1500), (70, 1300)]):
+        long_a = _words(rng, 6, "abcd", lo, hi)
+        col_a, col_b = dfa[v].to_numpy().copy(), dfb[v].to_numpy().copy()
+        for k, s in enumerate(long_a):
+            col_a[ra[k]] = s
+            t = list(s)
+            t[int(rng.integers(0, len(t)))] = "x"
+            col_b[rb[k]] = "".join(t) if k % 2 == 0 else s[:-1]
+        col_b[rb2[0]] = long_a[0]  # a long value alone in another block
+        dfa[v], dfb[v] = col_a, col_b
+    bl = BlockedLinkage(dfa, dfb, ba, bb, vf, vf, ["house_number"], ["house_number"])
+    ref, _ = per_block_loop(dfa, dfb, ba, bb, vf, ["house_number"])
+    bl.fit(use_graph=True)
+    np.testing.assert_array_equal(bl.Block_Counts, ref)
+    bl.fit(use_graph=True)
+    np.testing.assert_array_equal(bl.Block_Counts, ref, err_msg="second replay")
+    bl.run_per_block()
+    np.testing.assert_array_equal(bl.collect(), ref, err_msg="per-block launch loop")
+    _, _, counts = oracle.comparison_fit([dfa[v].to_numpy()[ra] for v in vf], [dfb[v].to_numpy()[rb] for v in vf],
+                                         [dfa["house_number"].to_numpy()[ra]], [dfb["house_number"].to_numpy()[rb]])
+    np.testing.assert_array_equal(bl.Block_Counts[b0], counts)
+    assert counts[1:].sum() > 0
 
 
 @pytest.mark.parametrize("mode,order", [("link", "random"), ("dedup", "random"), ("link", "sorted"),
