@@ -109,6 +109,8 @@ int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* t
 // the same sizes are caller-provided.
 int radix_sort_u64(uint64_t* keys, uint64_t* keys_tmp, uint64_t* vals, uint64_t* vals_tmp,
                    int64_t n, int begin_bit, int end_bit, hipStream_t s);
+// Sorts the keys `in` into `out` (in is overwritten; distinct buffers).
+int radix_sort_u64_into(uint64_t* in, uint64_t* out, int64_t n, int begin_bit, int end_bit, hipStream_t s);
 // Same for 32-bit keys with 64-bit values.
 int radix_sort_u32(uint32_t* keys, uint32_t* keys_tmp, uint64_t* vals, uint64_t* vals_tmp,
                    int64_t n, int begin_bit, int end_bit, hipStream_t s);

@@ -139,6 +139,96 @@ __global__ void __launch_bounds__(kThreads) k_expand_link(const uint64_t* __rest
   });
 }
 
+// ---------------------- linkage expansion in id order ----------------------
+// When values repeat, the record pairs of a unique A value ua are the same
+// (B row, level) list for each of its rows: the partner list P(ua), the rows
+// of every B value ub it hits, sorted. So the sorted record pairs are built
+// from the sorted partner lists (|rows_a(ua)| times smaller than the record
+// pairs) and one ordered copy per A row, instead of a radix sort of every
+// record pair.
+
+// Per hit: record pairs (|rows_a(ua)| * |rows_b(ub)|) and partner entries
+// (|rows_b(ub)|).
+__global__ void __launch_bounds__(kThreads) k_expand_link_count2(const uint64_t* __restrict__ hits, int64_t n_hits,
+                                                                 int64_t n_ub, const int64_t* __restrict__ starts_a,
+                                                                 const int64_t* __restrict__ starts_b,
+                                                                 uint64_t* __restrict__ cnt_full,
+                                                                 uint64_t* __restrict__ cnt_part) {
+  for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h < n_hits; h += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t id = hits[h] >> 2;
+    const uint64_t ua = id / (uint64_t)n_ub, ub = id % (uint64_t)n_ub;
+    const uint64_t cb = (uint64_t)(starts_b[ub + 1] - starts_b[ub]);
+    cnt_full[h] = (uint64_t)(starts_a[ua + 1] - starts_a[ua]) * cb;
+    cnt_part[h] = cb;
+  }
+}
+
+// Partner entries ((ua * n_b + rb) * 4 + level), unordered.
+__global__ void __launch_bounds__(kThreads) k_expand_partners(const uint64_t* __restrict__ hits, int64_t n_hits,
+                                                              int64_t n_ub, const int64_t* __restrict__ starts_b,
+                                                              const int64_t* __restrict__ rows_b, int64_t n_b,
+                                                              const uint64_t* __restrict__ offs, int64_t total,
+                                                              uint64_t* __restrict__ out) {
+  for_each_output(offs, n_hits, total, [&](int64_t o, int64_t h) {
+    const uint64_t key = hits[h];
+    const uint64_t id = key >> 2;
+    const uint64_t ua = id / (uint64_t)n_ub, ub = id % (uint64_t)n_ub;
+    const int64_t rb = rows_b[starts_b[ub] + (o - (int64_t)offs[h])];
+    out[o] = ((ua * (uint64_t)n_b + (uint64_t)rb) << 2) | (key & 3ull);
+  });
+}
+
+// Unique value of every A row (rows of value g: rows_a[starts_a[g] ..
+// starts_a[g + 1])).
+__global__ void __launch_bounds__(kThreads) k_row_values(const int64_t* __restrict__ starts_a,
+                                                         const int64_t* __restrict__ rows_a, int64_t n_ua,
+                                                         int64_t* __restrict__ value_of) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_ua; g += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t k = starts_a[g]; k < starts_a[g + 1]; ++k) value_of[rows_a[k]] = g;
+}
+
+// First entry of each value's partner list in the sorted partners
+// (value g's entries are [first[g], first[g + 1]); first[n_ua] = n_part).
+__global__ void __launch_bounds__(kThreads) k_partner_bounds(const uint64_t* __restrict__ part, int64_t n_part,
+                                                             int64_t n_ua, int64_t n_b, uint64_t* __restrict__ first) {
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g <= n_ua; g += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = ((uint64_t)g * (uint64_t)n_b) << 2;
+    int64_t lo = 0, hi = n_part;  // first index with part[i] >= key
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (part[mid] < key)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    first[g] = (uint64_t)lo;
+  }
+}
+
+// Record pairs per A row: its value's partner count.
+__global__ void __launch_bounds__(kThreads) k_row_pair_counts(const int64_t* __restrict__ value_of, int64_t n_a,
+                                                              const uint64_t* __restrict__ first,
+                                                              uint64_t* __restrict__ cnt) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n_a; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = value_of[r];
+    cnt[r] = g >= 0 ? first[g + 1] - first[g] : 0ull;
+  }
+}
+
+// The sorted record pairs: A rows in order, each followed by its value's
+// partner list with the row put in place of the value.
+__global__ void __launch_bounds__(kThreads) k_expand_rows(const int64_t* __restrict__ value_of, int64_t n_a,
+                                                          const uint64_t* __restrict__ first,
+                                                          const uint64_t* __restrict__ part, int64_t n_b,
+                                                          const uint64_t* __restrict__ offs, int64_t total,
+                                                          uint64_t* __restrict__ out) {
+  for_each_output(offs, n_a, total, [&](int64_t o, int64_t r) {
+    const uint64_t g = (uint64_t)value_of[r];
+    const uint64_t v = part[first[g] + (uint64_t)(o - (int64_t)offs[r])];
+    out[o] = v + (((uint64_t)r - g) * (uint64_t)n_b << 2);  // (r * n_b + rb) * 4 + level, mod 2^64
+  });
+}
+
 // -------------------------- deduplication expansion ------------------------
 // Hit index h < n_hits: a value pair; h >= n_hits: value u = h - n_hits paired
 // with itself (the reference's forced diagonal) at diag_level.
@@ -893,6 +983,70 @@ extern "C" int fh_expand_link(const uint64_t* hits, int64_t n_hits, int64_t n_ub
   if ((int64_t)total > capacity || total == 0) return FH_OK;
   hipLaunchKernelGGL(k_expand_link, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, hits, n_hits, n_ub,
                      starts_a, rows_a, starts_b, rows_b, n_b, cnt, (int64_t)total, out);
+  FH_HIP(hipGetLastError());
+  return FH_OK;
+}
+
+extern "C" int fh_expand_link_sorted(const uint64_t* hits, int64_t n_hits, int64_t n_ua, int64_t n_ub,
+                                     const int64_t* starts_a, const int64_t* rows_a, int64_t n_a,
+                                     const int64_t* starts_b, const int64_t* rows_b, int64_t n_b, uint64_t* out,
+                                     int64_t capacity, int64_t* n_out, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(n_out != nullptr, "fh_expand_link_sorted: n_out is null");
+  FH_REQUIRE(n_hits >= 0 && n_ua > 0 && n_ub > 0 && n_a > 0 && n_b > 0, "fh_expand_link_sorted: bad sizes");
+  *n_out = 0;
+  if (n_hits == 0) return FH_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Scratch scr(s);
+  uint64_t *cnt_full = nullptr, *cnt_part = nullptr, *tot = nullptr;
+  FH_TRY(scr.alloc(&cnt_full, (size_t)n_hits));
+  FH_TRY(scr.alloc(&cnt_part, (size_t)n_hits));
+  FH_TRY(scr.alloc(&tot, 2));
+  hipLaunchKernelGGL(k_expand_link_count2, dim3(grid_for(n_hits)), dim3(kThreads), 0, s, hits, n_hits, n_ub, starts_a,
+                     starts_b, cnt_full, cnt_part);
+  FH_HIP(hipGetLastError());
+  FH_TRY(exclusive_scan_u64(cnt_full, cnt_full, n_hits, tot, s));
+  FH_TRY(exclusive_scan_u64(cnt_part, cnt_part, n_hits, tot + 1, s));
+  uint64_t h_tot[2] = {0, 0};
+  FH_HIP(hipMemcpyAsync(h_tot, tot, sizeof(h_tot), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  const uint64_t total = h_tot[0], n_part = h_tot[1];
+  *n_out = (int64_t)total;
+  if ((int64_t)total > capacity || total == 0) return FH_OK;
+  const int bits = bits_for(((uint64_t)n_a * (uint64_t)n_b - 1) * 4 + 3);
+  if (total < 2 * n_part) {
+    // Few repeated A values: expand every record pair and sort them.
+    uint64_t* tmp = nullptr;
+    FH_TRY(scr.alloc(&tmp, (size_t)total));
+    hipLaunchKernelGGL(k_expand_link, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, hits, n_hits, n_ub,
+                       starts_a, rows_a, starts_b, rows_b, n_b, cnt_full, (int64_t)total, tmp);
+    FH_HIP(hipGetLastError());
+    return radix_sort_u64_into(tmp, out, (int64_t)total, 0, bits, s);
+  }
+  uint64_t *part = nullptr, *part_tmp = nullptr, *first = nullptr, *rcnt = nullptr;
+  int64_t* value_of = nullptr;
+  FH_TRY(scr.alloc(&part_tmp, (size_t)n_part));
+  FH_TRY(scr.alloc(&part, (size_t)n_part));
+  FH_TRY(scr.alloc(&first, (size_t)n_ua + 1));
+  FH_TRY(scr.alloc(&value_of, (size_t)n_a));
+  FH_TRY(scr.alloc(&rcnt, (size_t)n_a));
+  hipLaunchKernelGGL(k_expand_partners, dim3(grid_tiles((int64_t)n_part)), dim3(kThreads), 0, s, hits, n_hits, n_ub,
+                     starts_b, rows_b, n_b, cnt_part, (int64_t)n_part, part_tmp);
+  FH_HIP(hipGetLastError());
+  FH_TRY(radix_sort_u64_into(part_tmp, part, (int64_t)n_part, 0,
+                             bits_for(((uint64_t)n_ua * (uint64_t)n_b - 1) * 4 + 3), s));
+  // Rows without a value in the groups (not listed in rows_a) pair with nothing.
+  FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(value_of), ~0ull, n_a, s));
+  hipLaunchKernelGGL(k_row_values, dim3(grid_for(n_ua)), dim3(kThreads), 0, s, starts_a, rows_a, n_ua, value_of);
+  FH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_partner_bounds, dim3(grid_for(n_ua + 1)), dim3(kThreads), 0, s, part, (int64_t)n_part, n_ua,
+                     n_b, first);
+  FH_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_row_pair_counts, dim3(grid_for(n_a)), dim3(kThreads), 0, s, value_of, n_a, first, rcnt);
+  FH_HIP(hipGetLastError());
+  FH_TRY(exclusive_scan_u64(rcnt, rcnt, n_a, nullptr, s));
+  hipLaunchKernelGGL(k_expand_rows, dim3(grid_tiles((int64_t)total)), dim3(kThreads), 0, s, value_of, n_a, first,
+                     part, n_b, rcnt, (int64_t)total, out);
   FH_HIP(hipGetLastError());
   return FH_OK;
 }

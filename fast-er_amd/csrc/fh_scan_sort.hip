@@ -386,6 +386,10 @@ int radix_sort_u32_into(uint32_t* in, uint32_t* out, uint64_t* vin, uint64_t* vo
   return radix_sort_into_impl<uint32_t>(in, out, vin, vout, n, begin_bit, end_bit, s);
 }
 
+int radix_sort_u64_into(uint64_t* in, uint64_t* out, int64_t n, int begin_bit, int end_bit, hipStream_t s) {
+  return radix_sort_into_impl<uint64_t>(in, out, nullptr, nullptr, n, begin_bit, end_bit, s);
+}
+
 }  // namespace fh
 
 extern "C" int fh_sort_u64(uint64_t* keys, int64_t n, int begin_bit, int end_bit, fh_stream_t stream) {

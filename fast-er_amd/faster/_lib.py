@@ -66,6 +66,8 @@ _SIGS = {
                             ctypes.POINTER(_i64), ctypes.POINTER(JwStats), _vp]),
     "fh_expand_link": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _i64,
                               ctypes.POINTER(_i64), _vp]),
+    "fh_expand_link_sorted": (_i32, [_vp, _i64, _i64, _i64, _vp, _vp, _i64, _vp, _vp, _i64, _vp, _i64,
+                                     ctypes.POINTER(_i64), _vp]),
     "fh_expand_dedup": (_i32, [_vp, _i64, _i64, _vp, _vp, _i64, _i32, _vp, _i64,
                                ctypes.POINTER(_i64), _vp]),
     "fh_exact_link": (_i32, [_vp, _i64, _vp, _vp, _i64, _i64, _vp, _i64, ctypes.POINTER(_i64), _vp]),

@@ -89,6 +89,30 @@ def expand_link(hits, n_ub, starts_a, rows_a, starts_b, rows_b, n_b):
     return out[:need]
 
 
+def expand_link_sorted(hits, n_ua, n_ub, starts_a, rows_a, n_a, starts_b, rows_b, n_b):
+    """Record-pair level keys of the unique-value hits, ascending
+    (fh_expand_link_sorted)."""
+    L = _lib.load()
+    n_hits = hits.numel()
+    if n_hits == 0:
+        return _empty_i64(0)[:0]
+
+    def run(cap):
+        out = _empty_i64(cap)
+        n = ctypes.c_int64(0)
+        check(L.fh_expand_link_sorted(ptr(hits), _i64(n_hits), _i64(n_ua), _i64(n_ub), ptr(starts_a), ptr(rows_a),
+                                      _i64(n_a), ptr(starts_b), ptr(rows_b), _i64(n_b), ptr(out), _i64(cap),
+                                      ctypes.byref(n), stream()), "fh_expand_link_sorted")
+        return n.value, out
+
+    need, out = _retry(run, max(2 * n_hits, 1024, int(1.25 * _expand_rate[0] * n_hits)))
+    _expand_rate[0] = max(_expand_rate[0], need / n_hits)
+    return out[:need]
+
+
+_expand_rate = [0.0]  # largest record pairs per hit seen so far (sizes the next output)
+
+
 def expand_dedup(hits, n_u, starts, rows, n, diag_level):
     L = _lib.load()
     n_hits = hits.numel()

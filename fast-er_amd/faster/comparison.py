@@ -38,8 +38,7 @@ def fuzzy_keys_link(ga, gb, p=0.1, lower_thr=0.88, upper_thr=0.94, stats=None):
                               _lib.FH_MODE_LINK, p, lower_thr, upper_thr, want_stats=stats is not None)
     if stats is not None:
         stats.append(st)
-    keys = _ops.expand_link(hits, gb.n_unique, ga.starts, ga.rows, gb.starts, gb.rows, n_b)
-    return _ops.sort_keys(keys, _ops.max_pair_key(n_a * n_b))
+    return _ops.expand_link_sorted(hits, ga.n_unique, gb.n_unique, ga.starts, ga.rows, n_a, gb.starts, gb.rows, n_b)
 
 
 def exact_keys_link(group_a, starts_b, rows_b, n_groups, n_b):

@@ -257,6 +257,21 @@ int fh_expand_link(const uint64_t* hits, int64_t n_hits, int64_t n_ub, const int
                    fh_stream_t stream);
 
 /*
+ * The same expansion with the output keys ascending (the expansion plus the
+ * level-list sort, faster/comparison.py:429-458 and the cp.sort of the
+ * levels' indices). n_ua unique values on side A, n_a A rows (rows not listed
+ * in rows_a pair with nothing). When A values repeat (at least twice as many
+ * record pairs as (value, B row) pairs), each value's (B row, level) list is
+ * built and sorted once and copied per row in row order, instead of sorting
+ * every record pair. capacity protocol as fh_expand_link; out must not alias
+ * hits.
+ */
+int fh_expand_link_sorted(const uint64_t* hits, int64_t n_hits, int64_t n_ua, int64_t n_ub,
+                          const int64_t* starts_a, const int64_t* rows_a, int64_t n_a,
+                          const int64_t* starts_b, const int64_t* rows_b, int64_t n_b, uint64_t* out,
+                          int64_t capacity, int64_t* n_out, fh_stream_t stream);
+
+/*
  * Unique-value pair keys -> original-row pair keys (deduplication).
  *
  * Replaces the count and expansion kernels faster/deduplication.py:206-243

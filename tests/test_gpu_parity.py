@@ -163,6 +163,43 @@ def test_unique_link(oracle):
     _eq_lists(jaro_winkler_unique_gpu(A, B), oracle.jaro_winkler_unique(A, B), "unique")
 
 
+@pytest.mark.parametrize("n_ua,missing", [(40, False), (40, True), (2900, False)])
+def test_expand_link_sorted(n_ua, missing):
+    """fh_expand_link_sorted = every record pair of the unique-value hits,
+    ascending: with heavily repeated A values (the per-value partner lists
+    copied per row, rows without a value skipped) and with few repeats (the
+    expansion sorted directly), against NumPy."""
+    from faster import _lib, _ops
+
+    rng = np.random.default_rng(n_ua + missing)
+    dev = _lib.device()
+    n_a, n_b, n_ub = 3000, 2000, 300
+    val_a = rng.integers(0, n_ua, n_a)
+    if missing:
+        val_a[rng.random(n_a) < 0.1] = -1
+    val_b = rng.integers(0, n_ub, n_b)
+
+    def groups(vals, n_u):
+        rows = [np.nonzero(vals == u)[0] for u in range(n_u)]
+        starts = np.concatenate([[0], np.cumsum([len(r) for r in rows])]).astype(np.int64)
+        return starts, np.concatenate(rows).astype(np.int64), rows
+
+    sa, ra, rows_a = groups(val_a, n_ua)
+    sb, rb, rows_b = groups(val_b, n_ub)
+    ids = rng.choice(n_ua * n_ub, size=min(900, n_ua * n_ub // 3), replace=False)
+    lvl = rng.integers(1, 3, ids.size)
+    hits = (ids * 4 + lvl).astype(np.int64)
+    want = []
+    for i, l in zip(ids.tolist(), lvl.tolist()):
+        ua, ub = divmod(i, n_ub)
+        for x in rows_a[ua]:
+            want.append((int(x) * n_b + rows_b[ub]) * 4 + l)
+    want = np.sort(np.concatenate(want)) if want else np.zeros(0, np.int64)
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    got = _ops.expand_link_sorted(t(hits), n_ua, n_ub, t(sa), t(ra), n_a, t(sb), t(rb), n_b)
+    np.testing.assert_array_equal(got.cpu().numpy(), want)
+
+
 def test_exact_link(oracle):
     from faster.comparison import exact_gpu
 
