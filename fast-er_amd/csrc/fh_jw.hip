@@ -253,7 +253,7 @@ static int lrange_of(int len) { return len <= 16 ? 0 : (len <= 32 ? 1 : 2); }
 
 // Which (string-1 length range, tile kind) launches launch_kinds makes: all
 // with the per-lane kernel; only those the prefix form covers (string-1
-// values of up to 32 bytes against the packed and 32-bit kinds), with it;
+// values of up to 64 bytes against the packed and 32-bit kinds), with it;
 // or the rest, with the per-lane kernel.
 enum LaunchSet { kLaneAll, kPrefixOnly, kLaneRest };
 
@@ -267,12 +267,13 @@ static int launch_kinds(const JwArgs* args, size_t lds, const Plan& pl, int r, i
   for (int k = 0; k < kNumKinds; ++k) {
     const int64_t t0 = pl.tile_lo[k], nt = pl.tile_hi[k] - pl.tile_lo[k];
     if (nt <= 0) continue;
-    const bool covered = r < 2 && k != kW64;
+    const bool covered = k != kW64;
     if (set == kPrefixOnly) {
       if (!covered) continue;
       ++*launches;
       if (r == 0) FH_TRY(launch_jw_prefix_r0(k, args, lds_prefix, w0, nw, t0, nt, s));
       if (r == 1) FH_TRY(launch_jw_prefix_r1(k, args, lds_prefix, w0, nw, t0, nt, s));
+      if (r == 2) FH_TRY(launch_jw_prefix_r2(k, args, lds_prefix, w0, nw, t0, nt, s));
       continue;
     }
     if (set == kLaneRest && covered) continue;
@@ -285,11 +286,11 @@ static int launch_kinds(const JwArgs* args, size_t lds, const Plan& pl, int r, i
 }
 
 // Row codes of the sorted string-1 values [s_begin, s_end) (all of at most
-// 32 bytes) for the prefix form.
+// 64 bytes) for the prefix form.
 static int launch_prefix_codes(const uint8_t* chars, const int64_t* off, const uint64_t* sorted, int64_t s_begin,
                                int64_t s_end, const uint8_t* amap, int krows, uint32_t* codes, hipStream_t s) {
   if (s_end <= s_begin) return FH_OK;
-  const int64_t blocks = std::min<int64_t>(ceil_div((s_end - s_begin) * 8, 256), 8192);
+  const int64_t blocks = std::min<int64_t>(ceil_div((s_end - s_begin) * kPrefixCodeWords, 256), 8192);
   hipLaunchKernelGGL(k_prefix_codes, dim3((unsigned)blocks), dim3(256), 0, s, chars, off, sorted, s_begin, s_end, amap,
                      krows, codes);
   FH_HIP(hipGetLastError());
@@ -554,7 +555,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       }
     if (prefix) {
       unit_lo = pl.wave_lo[0];
-      unit_hi = pl.wave_hi[1];
+      unit_hi = pl.wave_hi[2];
     }
     unit_pairs.assign((size_t)(unit_hi - unit_lo) + 1, 0.0);
     for (int64_t u = unit_lo; u < unit_hi; ++u)
@@ -584,15 +585,16 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   uint32_t* d_codes = nullptr;
   unsigned long long* d_skipped = nullptr;  // per string-1 length: steps skipped per tile pass (stats)
   if (prefix) {
-    // Values of 1..32 bytes occupy sorted indices [start_a[1], start_a[33]).
-    FH_TRY(scr.alloc(&d_codes, (size_t)std::max<int64_t>(1, n_a) * 8));
-    FH_TRY(launch_prefix_codes(chars_a, offsets_a, ent_a, start_a[1], start_a[33], d_small, krows, d_codes, s));
-    if (stats && start_a[33] > start_a[1]) {
-      FH_TRY(scr.alloc(&d_skipped, 33));
-      FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(d_skipped), 0, 33, s));
-      const int64_t blocks = std::min<int64_t>(ceil_div(start_a[33] - start_a[1], 256), 4096);
+    // Values of 1..64 bytes occupy sorted indices [start_a[1], start_a[65]).
+    const int64_t c1 = start_a[1], c2 = start_a[kPrefixMaxLen + 1];
+    FH_TRY(scr.alloc(&d_codes, (size_t)std::max<int64_t>(1, n_a) * kPrefixCodeWords));
+    FH_TRY(launch_prefix_codes(chars_a, offsets_a, ent_a, c1, c2, d_small, krows, d_codes, s));
+    if (stats && c2 > c1) {
+      FH_TRY(scr.alloc(&d_skipped, kPrefixMaxLen + 1));
+      FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(d_skipped), 0, kPrefixMaxLen + 1, s));
+      const int64_t blocks = std::min<int64_t>(ceil_div(c2 - c1, 256), 4096);
       hipLaunchKernelGGL(k_prefix_restarts, dim3((unsigned)blocks), dim3(256), 0, s, d_codes,
-                         (const int64_t*)(d_cls + 2 * kNumClasses), start_a[1], start_a[33], d_skipped);
+                         (const int64_t*)(d_cls + 2 * kNumClasses), c1, c2, d_skipped);
       FH_HIP(hipGetLastError());
     }
   }
@@ -663,7 +665,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   int64_t n_candidates = 0;
   if (!bound) {
     // The per-lane kernel: everything, or with the prefix form what it does
-    // not cover (string-1 values longer than 32 bytes, kW64 tiles).
+    // not cover (string-2 values longer than 32 bytes: kW64 tiles).
     if (stats) FH_HIP(hipEventRecord(evr[0], s));
     const LaunchSet set = prefix ? kLaneRest : kLaneAll;
     for (int r = 0; r < 3; ++r)
@@ -705,7 +707,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
         FH_TRY(launch_jw_bound(mode, d_args, unit_lo + a0, a1 - a0, 0, nbt, s));
         ++launches;
       } else {
-        for (int r = 0; r < 2; ++r)
+        for (int r = 0; r < 3; ++r)
           FH_TRY(launch_kinds(d_args, lds_bytes, pl, r, unit_lo + a0, unit_lo + a1, s, &launches, kPrefixOnly,
                               lds_prefix));
       }
@@ -833,15 +835,16 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   }
   uint64_t h_count[2] = {0, 0};
   FH_HIP(hipMemcpyAsync(h_count, counter, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
-  std::vector<unsigned long long> h_skipped(33, 0);
+  std::vector<unsigned long long> h_skipped(kPrefixMaxLen + 1, 0);
   if (d_skipped)
-    FH_HIP(hipMemcpyAsync(h_skipped.data(), d_skipped, 33 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    FH_HIP(hipMemcpyAsync(h_skipped.data(), d_skipped, h_skipped.size() * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
   FH_HIP(hipStreamSynchronize(s));
   if (stats && d_skipped) {
     // Each skipped step saves three ops on every string-2 dword the value
     // meets: 3 / L lane-ops per pair of a kind with L strings per dword.
     double skipped_ops = 0;
-    for (int la = 1; la <= 32; ++la) {
+    for (int la = 1; la <= kPrefixMaxLen; ++la) {
       if (!h_skipped[la]) continue;
       double per_step = 0;
       for (int lb = 1; lb <= 32; ++lb) {

@@ -39,11 +39,17 @@ int launch_jw_prefix_r0(int kind, const JwArgs* args, size_t lds_bytes, int64_t 
                         int64_t nt, hipStream_t s);
 int launch_jw_prefix_r1(int kind, const JwArgs* args, size_t lds_bytes, int64_t w0, int64_t nw, int64_t t0,
                         int64_t nt, hipStream_t s);
+// The same for string-1 lengths 33..64 (r2): one rolled pass for every
+// length (these values are rare; their steps are not unrolled).
+int launch_jw_prefix_r2(int kind, const JwArgs* args, size_t lds_bytes, int64_t w0, int64_t nw, int64_t t0,
+                        int64_t nt, hipStream_t s);
 // Scores the first n entries of args->cand (prefix-form survivors: both
 // values of at most 32 bytes) and appends their level keys.
 int launch_eval_prefix(const JwArgs* args, int64_t n, hipStream_t s);
 
 constexpr int kPrefixTileWidth = 256;  // dwords: 64 lanes x 4
+constexpr int kPrefixMaxLen = 64;      // longest string-1 value of the prefix form
+constexpr int kPrefixCodeWords = 16;   // row-code dwords per string-1 value (4 codes each)
 // LDS bytes of one workgroup of the prefix form: the tile (rows of 256
 // dwords, no padding: every lane reads the same row) and the survivor queues.
 inline size_t jw_prefix_lds_bytes(int krows) {
@@ -262,6 +268,83 @@ __device__ __forceinline__ void prefix_pass(const JwArgs& a, PassCtx& c, int cnt
   }
 }
 
+// Complemented window of string-1 position i (run-time form of prefix_nwin).
+template <int KIND>
+__device__ __forceinline__ uint32_t nwin_rt(int i, int w, int lenB) {
+  const int lo = max(0, i - w);
+  const int hi = min(lenB - 1, i + w);
+  const int width = hi - lo + 1;
+  uint32_t win = width <= 0 ? 0u : (width >= 32 ? ~0u : ((1u << width) - 1u)) << lo;
+  if constexpr (packed_kind(KIND)) win *= kind_rep(KIND);
+  return __builtin_amdgcn_readfirstlane(~win);
+}
+
+// prefix_pass for string-1 values of 33..64 bytes (rare), rolled: each
+// step takes its row code from the value's code words with a scalar load
+// and computes its window on the scalar unit.
+template <int KIND>
+__device__ __forceinline__ void prefix_pass_long(const JwArgs& a, PassCtx& c, int cntA, int LA) {
+  constexpr bool PK = packed_kind(KIND);
+  constexpr int NL = PK ? kind_lanes(KIND) : 1;
+  constexpr uint32_t ONES = PK ? kind_ones(KIND) : 1u;
+  constexpr int DE = kPrefixDepth;  // LA > 32 > DE
+  const int lenB = c.lenB;
+  const int w = c.w;
+  const int lane = (int)__lane_id();
+  const uint32_t lane_addr = lds_addr(c.lds) + (uint32_t)lane * 16u;
+  const int jlane = 4 * NL * lane;
+  const uint32_t valid = low_bits(c.cntB - jlane);
+  const int mm = c.mm;
+  uint32_t st[DE][4];
+  const uint32_t zero[4] = {0u, 0u, 0u, 0u};
+  uint32_t prev0 = 0;
+#pragma unroll 1
+  for (int s = 0; s < cntA; ++s) {
+    const uint32_t* cp = a.codes_a + (size_t)(c.s0a + s) * kPrefixCodeWords;
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(cp[0]);
+    int j0 = 0;
+    if (s > 0) {
+      const uint32_t x = w0 ^ prev0;
+      j0 = x ? (int)(__builtin_ctz(x) >> 3) : 4;
+      j0 = j0 < DE ? j0 : DE;
+    }
+    prev0 = w0;
+    if (j0 == 0) prefix_step<ONES>(prefix_row(lane_addr, w0 & 0xFFu), nwin_rt<KIND>(0, w, lenB), zero, st[0]);
+    if (j0 <= 1)
+      prefix_step<ONES>(prefix_row(lane_addr, (w0 >> 8) & 0xFFu), nwin_rt<KIND>(1, w, lenB), st[0], st[1]);
+    if (j0 <= 2)
+      prefix_step<ONES>(prefix_row(lane_addr, (w0 >> 16) & 0xFFu), nwin_rt<KIND>(2, w, lenB), st[1], st[2]);
+    uint32_t na[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) na[u] = st[DE - 1][u];
+#pragma unroll 1
+    for (int i = DE; i < LA; ++i) {
+      const uint32_t wd = __builtin_amdgcn_readfirstlane(cp[i >> 2]);
+      prefix_step<ONES>(prefix_row(lane_addr, (wd >> (8 * (i & 3))) & 0xFFu), nwin_rt<KIND>(i, w, lenB), na, na);
+    }
+    uint32_t mx = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < NL; ++k) mx = max(mx, prefix_count<KIND>(na[u], k));
+    if (__ballot((int)mx >= mm) == 0) continue;
+    uint32_t pass = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int k = 0; k < NL; ++k) pass |= (uint32_t)((int)prefix_count<KIND>(na[u], k) >= mm) << (u * NL + k);
+    pass &= valid;
+    if (c.diag) pass &= low_bits(c.s0a + s - c.s0b - jlane);
+#pragma unroll 1
+    while (__ballot(pass != 0u) != 0) {
+      const bool pred = pass != 0u;
+      const int b = pred ? __builtin_ctz(pass) : 0;
+      prefix_push(a, c, pred, (uint64_t)(c.s0b + jlane + b), s);
+      pass &= pass - 1u;
+    }
+  }
+}
+
 template <int LO, int HI, int KIND, int CW>
 struct PrefixDispatch {
   __device__ static __forceinline__ void run(int lenA, const JwArgs& a, PassCtx& c, int cntA,
@@ -282,7 +365,7 @@ struct PrefixDispatch<HI, HI, KIND, CW> {
 
 // This workgroup's waves [wave_first, wave_first + kWaves) ∩ [.., wave_end)
 // against tiles [t_begin, t_end) of kind KIND.
-template <int LMIN, int LMAX, int KIND>
+template <int LMIN, int LMAX, int KIND, bool LONG>
 __device__ __forceinline__ void run_item_prefix(const JwArgs& a, uint32_t* lds, int64_t wave_first, int64_t wave_end,
                                                 int64_t t_begin, int64_t t_end) {
   const int krows = a.krows;
@@ -307,10 +390,11 @@ __device__ __forceinline__ void run_item_prefix(const JwArgs& a, uint32_t* lds, 
   c.s0a = uniform_i64(wm.s0);
 
   // Row codes of string 1 number `lane` (four per dword).
-  constexpr int CW = (LMAX + 3) / 4;
+  constexpr int CW = LONG ? 1 : (LMAX + 3) / 4;
   uint32_t codes[CW];
 #pragma unroll
-  for (int k = 0; k < CW; ++k) codes[k] = c.lane_ok ? a.codes_a[(size_t)(c.s0a + lane) * 8 + k] : 0u;
+  for (int k = 0; k < CW; ++k)
+    codes[k] = (LONG || !c.lane_ok) ? 0u : a.codes_a[(size_t)(c.s0a + lane) * kPrefixCodeWords + k];
 
   for (int64_t t = t_begin; t < t_end; ++t) {
     const TileMeta tm = a.tiles[t];
@@ -337,23 +421,26 @@ __device__ __forceinline__ void run_item_prefix(const JwArgs& a, uint32_t* lds, 
     c.mm = a.mmin[lenA * (kLongLen + 1) + c.lenB];
     if (c.mm > min(lenA, c.lenB)) continue;
     c.w = max(lenA, c.lenB) / 2 - 1;
-    PrefixDispatch<LMIN, LMAX, KIND, CW>::run(lenA, a, c, cntA, codes);
+    if constexpr (LONG)
+      prefix_pass_long<KIND>(a, c, cntA, lenA);
+    else
+      PrefixDispatch<LMIN, LMAX, KIND, CW>::run(lenA, a, c, cntA, codes);
   }
   if (wave_ok && c.qn > 0) prefix_flush(a, c.q, c.qn, (uint64_t)wm.s0);
 }
 
 // Row codes of the string-1 values [s_begin, s_end) in sorted order (every
-// value of at most 32 bytes), four per dword; positions past the value's
+// value of at most kPrefixMaxLen bytes), four per dword; positions past the value's
 // length hold the all-blocked row krows - 1.
 __global__ void __launch_bounds__(256) k_prefix_codes(const uint8_t* __restrict__ chars,
                                                       const int64_t* __restrict__ off,
                                                       const uint64_t* __restrict__ sorted, int64_t s_begin,
                                                       int64_t s_end, const uint8_t* __restrict__ amap, int krows,
                                                       uint32_t* __restrict__ codes) {
-  const int64_t n = (s_end - s_begin) * 8;
+  const int64_t n = (s_end - s_begin) * kPrefixCodeWords;
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t si = s_begin + (g >> 3);
-    const int q = (int)(g & 7);
+    const int64_t si = s_begin + g / kPrefixCodeWords;
+    const int q = (int)(g % kPrefixCodeWords);
     const int64_t e = (int64_t)sorted[si];
     const int64_t o = off[e];
     const int len = (int)(off[e + 1] - o);
@@ -364,7 +451,7 @@ __global__ void __launch_bounds__(256) k_prefix_codes(const uint8_t* __restrict_
       const uint32_t code = i < len ? (uint32_t)amap[chars[o + i]] : (uint32_t)(krows - 1);
       w |= code << (8 * k);
     }
-    codes[si * 8 + q] = w;
+    codes[si * kPrefixCodeWords + q] = w;
   }
 }
 
@@ -391,21 +478,21 @@ __global__ void __launch_bounds__(256) k_prefix_restarts(const uint32_t* __restr
                                                          const int64_t* __restrict__ cls_start_a, int64_t s_begin,
                                                          int64_t s_end, unsigned long long* __restrict__ skipped) {
   __shared__ int64_t starts[kNumClasses];
-  __shared__ unsigned long long h[33];
+  __shared__ unsigned long long h[kPrefixMaxLen + 1];
   for (int i = threadIdx.x; i < kNumClasses; i += blockDim.x) starts[i] = cls_start_a[i];
-  for (int i = threadIdx.x; i < 33; i += blockDim.x) h[i] = 0;
+  for (int i = threadIdx.x; i <= kPrefixMaxLen; i += blockDim.x) h[i] = 0;
   __syncthreads();
   for (int64_t si = s_begin + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; si < s_end;
        si += (int64_t)gridDim.x * blockDim.x) {
     const int la = class_of(starts, si);
-    if (la < 1 || la > 32 || (si - starts[la]) % 64 == 0) continue;
-    const uint32_t x = codes[si * 8] ^ codes[(si - 1) * 8];
+    if (la < 1 || la > kPrefixMaxLen || (si - starts[la]) % 64 == 0) continue;
+    const uint32_t x = codes[si * kPrefixCodeWords] ^ codes[(si - 1) * kPrefixCodeWords];
     const int depth = min(kPrefixDepth, la - 1);
     const int j0 = min(x ? (int)(__builtin_ctz(x) >> 3) : 4, depth);
     if (j0 > 0) atomicAdd(&h[la], (unsigned long long)j0);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < 33; i += blockDim.x)
+  for (int i = threadIdx.x; i <= kPrefixMaxLen; i += blockDim.x)
     if (h[i]) atomicAdd(&skipped[i], h[i]);
 }
 
@@ -423,6 +510,9 @@ __device__ __forceinline__ int prefix_level(const JwArgs& a, const int64_t* star
   y = eb;
   const int la = class_of(starts_a, sa);
   const int lb = class_of(starts_b, sb);
+  // String-1 values over 32 bytes: the table scorer (its own string order
+  // test for deduplication).
+  if (la > 32) return candidate_level<true>(a, sa, sb, x, y);
   if (a.mode == FH_MODE_DEDUP) {
     const int64_t oa = a.off_a[ea], ob = a.off_b[eb];
     if (!lex_greater(a.chars_a + oa, la, a.chars_b + ob, lb)) {
@@ -445,7 +535,7 @@ __device__ __forceinline__ int prefix_level(const JwArgs& a, const int64_t* star
   }
   const uint32_t* base = a.table + (size_t)tile * a.krows * a.tw + col;
   const uint32_t lmask = lb >= 32 ? ~0u : ((1u << lb) - 1u);
-  const uint32_t* cp = a.codes_a + (size_t)sa * 8;
+  const uint32_t* cp = a.codes_a + (size_t)sa * kPrefixCodeWords;
   uint32_t cw[8];
 #pragma unroll
   for (int k = 0; k < 8; ++k) cw[k] = cp[k];
@@ -522,7 +612,8 @@ k_jw_prefix(const JwArgs* __restrict__ ap, int64_t wave0, int64_t nwaves, int64_
   extern __shared__ uint32_t lds[];
   const int64_t t_begin = tile0 + (int64_t)blockIdx.y * tiles_per_chunk;
   const int64_t t_end = min(t_begin + tiles_per_chunk, tile0 + ntiles);
-  run_item_prefix<LMIN, LMAX, KIND>(*ap, lds, wave0 + (int64_t)blockIdx.x * kWaves, wave0 + nwaves, t_begin, t_end);
+  run_item_prefix<LMIN, LMAX, KIND, (LMIN > 32)>(*ap, lds, wave0 + (int64_t)blockIdx.x * kWaves, wave0 + nwaves,
+                                                 t_begin, t_end);
 }
 
 template <int LMIN, int LMAX, int KIND>

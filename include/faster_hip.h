@@ -96,8 +96,8 @@ typedef struct {
  *   FH_JW_METHOD_PREFIX (default): the same bit-parallel match with the
  *     string-1 values walked one after another by each wave, whose lanes
  *     cover the string-2 tile; the match state of a prefix shared with the
- *     previous (sorted) value is reused. Values longer than 32 bytes and
- *     string-2 values longer than 32 bytes use the TILES kernel.
+ *     previous (sorted) value is reused. String-2 values longer than 32
+ *     bytes (and values over 64 bytes) use the TILES kernel.
  * No counterpart in the reference (one CUDA kernel scores every pair,
  * faster/comparison.py:133-186); the switch exists to time and check the
  * forms against each other.
