@@ -48,9 +48,16 @@ void clear_error();
 // Stream-ordered device scratch owned by one entry-point call. Memory comes
 // from HIP's stream-ordered pool and is returned on the same stream when the
 // object goes out of scope.
+// Keeps up to kPoolKeepBytes of freed scratch in the current device's
+// stream-ordered pool instead of returning it to the driver at every
+// synchronisation (the pool's default), so that the scratch of repeated
+// calls is served from the pool. Once per device per process.
+void keep_pool_memory();
+constexpr uint64_t kPoolKeepBytes = (uint64_t)16 << 30;
+
 class Scratch {
  public:
-  explicit Scratch(hipStream_t s) : stream_(s) {}
+  explicit Scratch(hipStream_t s) : stream_(s) { keep_pool_memory(); }
   ~Scratch() { release(); }
   Scratch(const Scratch&) = delete;
   Scratch& operator=(const Scratch&) = delete;

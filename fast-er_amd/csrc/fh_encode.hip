@@ -12,6 +12,7 @@
 // and rows of one value stay ascending because every pass is stable and the
 // sort starts from the identity permutation.
 #include <algorithm>
+#include <vector>
 
 #include "fh_common.h"
 
@@ -200,6 +201,165 @@ __global__ void __launch_bounds__(kEncThreads) k_side_starts(const int64_t* __re
   if (blockIdx.x == 0 && threadIdx.x == 0) starts_b[n_groups] = (int64_t)*total_b;
 }
 
+// ------------------- several string columns in one sort --------------------
+// The rows of k columns form one index space (column c's row r is global
+// index base[c] + r). One LSD sort over (column, string) orders every
+// column's rows at once, so k columns take the launches of one.
+constexpr int kMaxCols = 16;
+struct ColSet {
+  const uint8_t* chars[kMaxCols];
+  const int64_t* off[kMaxCols];
+  int64_t base[kMaxCols + 1];  // global index of each column's first row; base[k] = total rows
+  int k;
+};
+struct ColOut {
+  uint8_t* uchars[kMaxCols];
+  int64_t* uoff[kMaxCols];
+  int64_t* starts[kMaxCols];
+  int64_t* rows[kMaxCols];
+};
+
+__device__ __forceinline__ int col_of(const ColSet& cs, int64_t g) {
+  int c = 0;
+  while (c + 1 < cs.k && g >= cs.base[c + 1]) ++c;
+  return c;
+}
+
+__global__ void __launch_bounds__(kEncThreads) k_encm_init(ColSet cs, uint64_t* __restrict__ perm,
+                                                           uint64_t* __restrict__ lens,
+                                                           unsigned long long* __restrict__ max_len) {
+  unsigned long long m = 0;
+  const int64_t n = cs.base[cs.k];
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
+    const int c = col_of(cs, g);
+    const int64_t r = g - cs.base[c];
+    const uint64_t len = (uint64_t)(cs.off[c][r + 1] - cs.off[c][r]);
+    perm[g] = (uint64_t)g;
+    lens[g] = len;
+    m = m > len ? m : len;
+  }
+  for (int d = 32; d >= 1; d >>= 1) {
+    const unsigned long long o = __shfl_xor(m, d);
+    m = m > o ? m : o;
+  }
+  if (__lane_id() == 0) atomicMax(max_len, m);
+}
+
+// keys[i] = bytes [8 * word, 8 * word + 8) of global row perm[i] (zero-padded,
+// first byte most significant), or its column when word < 0.
+__global__ void __launch_bounds__(kEncThreads) k_encm_keys(ColSet cs, const uint64_t* __restrict__ perm, int word,
+                                                           uint64_t* __restrict__ keys) {
+  const int64_t n = cs.base[cs.k];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = (int64_t)perm[i];
+    const int c = col_of(cs, g);
+    if (word < 0) {
+      keys[i] = (uint64_t)c;
+      continue;
+    }
+    const int64_t r = g - cs.base[c];
+    const int64_t o = cs.off[c][r];
+    const int64_t len = cs.off[c][r + 1] - o;
+    const int64_t b0 = (int64_t)word * 8;
+    uint64_t w = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint64_t byte = b0 + k < len ? (uint64_t)cs.chars[c][o + b0 + k] : 0ull;
+      w = (w << 8) | byte;
+    }
+    keys[i] = w;
+  }
+}
+
+// After the sort, column c's rows occupy sorted positions [base[c],
+// base[c + 1]). heads[i] = 1 where position i starts a new value (of its
+// column); rows of column c get their local ids.
+__global__ void __launch_bounds__(kEncThreads) k_encm_heads(ColSet cs, ColOut out, const uint64_t* __restrict__ perm,
+                                                            uint64_t* __restrict__ heads) {
+  const int64_t n = cs.base[cs.k];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = col_of(cs, i);
+    const int64_t r = (int64_t)perm[i] - cs.base[c];
+    out.rows[c][i - cs.base[c]] = r;
+    bool head = i == cs.base[c];
+    if (!head) {
+      const int64_t q = (int64_t)perm[i - 1] - cs.base[c];
+      const int64_t* off = cs.off[c];
+      const uint8_t* ch = cs.chars[c];
+      const int64_t ox = off[r], oy = off[q];
+      const int64_t len = off[r + 1] - ox;
+      head = len != off[q + 1] - oy;
+      for (int64_t k = 0; !head && k < len; ++k) head = ch[ox + k] != ch[oy + k];
+    }
+    heads[i] = head ? 1ull : 0ull;
+  }
+}
+
+// First value id of every column (gid at its first sorted position; the
+// value count for an empty trailing column).
+__device__ __forceinline__ uint64_t col_g0(const ColSet& cs, const uint64_t* gid, const uint64_t* total, int c) {
+  return cs.base[c] < cs.base[cs.k] ? gid[cs.base[c]] : *total;
+}
+
+// starts and the unique values' lengths (global value id g = gid[i]).
+__global__ void __launch_bounds__(kEncThreads) k_encm_emit(ColSet cs, ColOut out, const uint64_t* __restrict__ perm,
+                                                           const uint64_t* __restrict__ heads,
+                                                           const uint64_t* __restrict__ gid,
+                                                           const uint64_t* __restrict__ n_values,
+                                                           uint64_t* __restrict__ ulen) {
+  const int64_t n = cs.base[cs.k];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (!heads[i]) continue;
+    const int c = col_of(cs, i);
+    const uint64_t g = gid[i];
+    const int64_t r = (int64_t)perm[i] - cs.base[c];
+    out.starts[c][g - col_g0(cs, gid, n_values, c)] = i - cs.base[c];
+    ulen[g] = (uint64_t)(cs.off[c][r + 1] - cs.off[c][r]);
+  }
+  if (blockIdx.x == 0 && (int)threadIdx.x < cs.k) {
+    const int c = (int)threadIdx.x;
+    const uint64_t g0 = col_g0(cs, gid, n_values, c);
+    const uint64_t g1 = c + 1 < cs.k ? col_g0(cs, gid, n_values, c + 1) : *n_values;
+    out.starts[c][g1 - g0] = cs.base[c + 1] - cs.base[c];
+  }
+}
+
+// Per value: its column's offsets (uoff relative to the column's first
+// value) and bytes; per column (first k threads): closing offset, value
+// count and byte count into counts[2c], counts[2c + 1].
+__global__ void __launch_bounds__(kEncThreads) k_encm_values(ColSet cs, ColOut out, const uint64_t* __restrict__ perm,
+                                                             const uint64_t* __restrict__ gid,
+                                                             const uint64_t* __restrict__ n_values,
+                                                             const uint64_t* __restrict__ goff,
+                                                             const uint64_t* __restrict__ total_bytes,
+                                                             int64_t* __restrict__ counts) {
+  const uint64_t nv = *n_values;
+  __shared__ uint64_t g0[kMaxCols + 1];
+  if ((int)threadIdx.x <= cs.k)
+    g0[threadIdx.x] = (int)threadIdx.x < cs.k ? col_g0(cs, gid, n_values, (int)threadIdx.x) : nv;
+  __syncthreads();
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < nv; g += (uint64_t)gridDim.x * blockDim.x) {
+    int c = 0;
+    while (c + 1 < cs.k && g >= g0[c + 1]) ++c;
+    const uint64_t gl = g - g0[c];
+    const uint64_t o = goff[g] - goff[g0[c]];
+    out.uoff[c][gl] = (int64_t)o;
+    const int64_t pos = out.starts[c][gl] + cs.base[c];
+    const int64_t r = (int64_t)perm[pos] - cs.base[c];
+    const int64_t so = cs.off[c][r];
+    const int64_t len = cs.off[c][r + 1] - so;
+    for (int64_t k = 0; k < len; ++k) out.uchars[c][o + k] = cs.chars[c][so + k];
+  }
+  if (blockIdx.x == 0 && (int)threadIdx.x < cs.k) {
+    const int c = (int)threadIdx.x;
+    const uint64_t e = g0[c + 1] < nv ? goff[g0[c + 1]] : *total_bytes;
+    const uint64_t b = g0[c] < nv ? goff[g0[c]] : *total_bytes;
+    out.uoff[c][g0[c + 1] - g0[c]] = (int64_t)(e - b);
+    counts[2 * c] = (int64_t)(g0[c + 1] - g0[c]);
+    counts[2 * c + 1] = (int64_t)(e - b);
+  }
+}
+
 }  // namespace
 }  // namespace fh
 
@@ -274,6 +434,97 @@ extern "C" int fh_group_strings(const uint8_t* chars, const int64_t* offsets, in
   FH_HIP(hipStreamSynchronize(s));
   *n_unique = (int64_t)nu;
   *unique_bytes = (int64_t)ub;
+  return FH_OK;
+}
+
+extern "C" int fh_group_strings_multi(int k, const uint8_t* const* chars, const int64_t* const* offsets,
+                                      const int64_t* n, uint8_t* const* uchars, int64_t* const* uoffsets,
+                                      int64_t* const* starts, int64_t* const* rows, int64_t* n_unique,
+                                      int64_t* unique_bytes, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(k >= 1 && k <= kMaxCols, "fh_group_strings_multi: 1..%d columns", kMaxCols);
+  FH_REQUIRE(n_unique && unique_bytes, "fh_group_strings_multi: null output count");
+  hipStream_t s = (hipStream_t)stream;
+  ColSet cs{};
+  ColOut out{};
+  cs.k = k;
+  cs.base[0] = 0;
+  for (int c = 0; c < k; ++c) {
+    FH_REQUIRE(n[c] >= 0, "fh_group_strings_multi: column %d has n < 0", c);
+    cs.chars[c] = chars[c];
+    cs.off[c] = offsets[c];
+    cs.base[c + 1] = cs.base[c] + n[c];
+    out.uchars[c] = uchars[c];
+    out.uoff[c] = uoffsets[c];
+    out.starts[c] = starts[c];
+    out.rows[c] = rows[c];
+  }
+  const int64_t N = cs.base[k];
+  FH_REQUIRE(N < (int64_t)0xFFFFFFFF, "fh_group_strings_multi: %lld rows exceed 2^32 - 1", (long long)N);
+  if (N == 0) {
+    const int64_t zero = 0;
+    for (int c = 0; c < k; ++c) {
+      FH_HIP(hipMemcpyAsync(uoffsets[c], &zero, sizeof(int64_t), hipMemcpyHostToDevice, s));
+      FH_HIP(hipMemcpyAsync(starts[c], &zero, sizeof(int64_t), hipMemcpyHostToDevice, s));
+      n_unique[c] = 0;
+      unique_bytes[c] = 0;
+    }
+    FH_HIP(hipStreamSynchronize(s));
+    return FH_OK;
+  }
+  Scratch scr(s);
+  uint64_t *perm = nullptr, *perm_tmp = nullptr, *keys = nullptr, *keys_tmp = nullptr, *small = nullptr;
+  int64_t* counts = nullptr;
+  FH_TRY(scr.alloc(&perm, (size_t)N));
+  FH_TRY(scr.alloc(&perm_tmp, (size_t)N));
+  FH_TRY(scr.alloc(&keys, (size_t)N + 1));
+  FH_TRY(scr.alloc(&keys_tmp, (size_t)N + 1));
+  FH_TRY(scr.alloc(&small, 4));  // [0] max length, [1] value count, [2] value bytes
+  FH_TRY(scr.alloc(&counts, 2 * (size_t)k));
+  FH_TRY(fill_u64(small, 0, 4, s));
+  const unsigned grid = grid_for(N);
+  hipLaunchKernelGGL(k_encm_init, dim3(grid), dim3(kEncThreads), 0, s, cs, perm, keys,
+                     reinterpret_cast<unsigned long long*>(small));
+  FH_HIP(hipGetLastError());
+  uint64_t max_len = 0;
+  FH_HIP(hipMemcpyAsync(&max_len, small, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  // Length, then the words from the last to the first, then the column.
+  FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, N, 0, std::max(1, bits_for(max_len)), s));
+  const int words = (int)ceil_div((int64_t)max_len, 8);
+  for (int w = words - 1; w >= 0; --w) {
+    hipLaunchKernelGGL(k_encm_keys, dim3(grid), dim3(kEncThreads), 0, s, cs, perm, w, keys);
+    FH_HIP(hipGetLastError());
+    const int valid_bytes = (int)std::min<int64_t>(8, (int64_t)max_len - 8 * (int64_t)w);
+    FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, N, 8 * (8 - valid_bytes), 64, s));
+  }
+  if (k > 1) {
+    hipLaunchKernelGGL(k_encm_keys, dim3(grid), dim3(kEncThreads), 0, s, cs, perm, -1, keys);
+    FH_HIP(hipGetLastError());
+    FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, N, 0, bits_for((uint64_t)k - 1), s));
+  }
+  uint64_t* heads = keys;
+  uint64_t* gid = keys_tmp;
+  hipLaunchKernelGGL(k_encm_heads, dim3(grid), dim3(kEncThreads), 0, s, cs, out, perm, heads);
+  FH_HIP(hipGetLastError());
+  FH_TRY(exclusive_scan_u64(heads, gid, N, small + 1, s));
+  uint64_t* ulen = perm_tmp;  // value lengths, then their offsets in place
+  FH_TRY(fill_u64(ulen, 0, N, s));
+  hipLaunchKernelGGL(k_encm_emit, dim3(grid), dim3(kEncThreads), 0, s, cs, out, perm, heads, gid, small + 1, ulen);
+  FH_HIP(hipGetLastError());
+  // Up to N values: the entries past the value count are 0, so the scan's
+  // total is the unique bytes.
+  FH_TRY(exclusive_scan_u64(ulen, ulen, N, small + 2, s));
+  hipLaunchKernelGGL(k_encm_values, dim3(grid), dim3(kEncThreads), 0, s, cs, out, perm, gid, small + 1, ulen,
+                     small + 2, counts);
+  FH_HIP(hipGetLastError());
+  std::vector<int64_t> h(2 * (size_t)k);
+  FH_HIP(hipMemcpyAsync(h.data(), counts, h.size() * sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  FH_HIP(hipStreamSynchronize(s));
+  for (int c = 0; c < k; ++c) {
+    n_unique[c] = h[2 * c];
+    unique_bytes[c] = h[2 * c + 1];
+  }
   return FH_OK;
 }
 

@@ -4,7 +4,10 @@
 // deduplication copy faster/deduplication.py:132-196) and its thresholding
 // and compaction (faster/comparison.py:321-338). The comparison kernel and
 // its design are in fh_jw_core.h.
+#include <array>
 #include <atomic>
+#include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 
@@ -18,6 +21,7 @@ std::atomic<int> g_jw_method{FH_JW_METHOD_PREFIX};
 // by 5% per call (sizes the next call's candidate buffer; any value is
 // correct, a large enough one avoids repeated rounds).
 std::atomic<double> g_prefix_rate{4e-4};
+
 
 // Candidate buffers of the prefix form, kept between calls: a call leases
 // one (concurrent calls on other streams lease others), grows it when its
@@ -42,18 +46,31 @@ class CandLease {
     std::lock_guard<std::mutex> lock(g_cand_mu);
     g_cand_pool.push_back(buf_);
   }
-  // Takes a buffer of this device from the pool (or an empty one) and makes
-  // it hold at least `need` entries.
+  // Takes a buffer of this device from the pool (the smallest that holds
+  // `need` entries, else the largest, else an empty one) and makes it hold at
+  // least `need` entries. Taking the best fit keeps concurrent callers with
+  // different needs from trading buffers and reallocating them (hipFree
+  // waits for the whole device).
   int acquire(int64_t need, hipStream_t s) {
     int dev = 0;
     FH_HIP(hipGetDevice(&dev));
     {
       std::lock_guard<std::mutex> lock(g_cand_mu);
+      int best = -1;
       for (size_t i = 0; i < g_cand_pool.size(); ++i) {
-        if (g_cand_pool[i].dev != dev) continue;
-        buf_ = g_cand_pool[i];
-        g_cand_pool.erase(g_cand_pool.begin() + (std::ptrdiff_t)i);
-        break;
+        const CandBuffer& b = g_cand_pool[i];
+        if (b.dev != dev) continue;
+        if (best < 0) {
+          best = (int)i;
+          continue;
+        }
+        const CandBuffer& c = g_cand_pool[(size_t)best];
+        const bool b_fits = b.cap >= need, c_fits = c.cap >= need;
+        if (b_fits != c_fits ? b_fits : (b_fits ? b.cap < c.cap : b.cap > c.cap)) best = (int)i;
+      }
+      if (best >= 0) {
+        buf_ = g_cand_pool[(size_t)best];
+        g_cand_pool.erase(g_cand_pool.begin() + best);
       }
     }
     buf_.dev = dev;
@@ -90,6 +107,23 @@ class CandLease {
   bool held_ = false;
 };
 }  // namespace
+const std::vector<uint8_t>& mmin_table(float p, float lo, float hi) {
+  static std::mutex mu;
+  static std::map<std::array<uint32_t, 3>, std::unique_ptr<std::vector<uint8_t>>> cache;
+  std::array<uint32_t, 3> key;
+  std::memcpy(&key[0], &p, 4);
+  std::memcpy(&key[1], &lo, 4);
+  std::memcpy(&key[2], &hi, 4);
+  std::lock_guard<std::mutex> lock(mu);
+  auto& slot = cache[key];
+  if (!slot) {
+    auto t = std::make_unique<std::vector<uint8_t>>((kLongLen + 1) * (kLongLen + 1), 255);
+    for (int la = 1; la <= kLongLen; ++la)
+      for (int lb = 1; lb <= kLongLen; ++lb) (*t)[la * (kLongLen + 1) + lb] = host_mmin(la, lb, p, lo, hi);
+    slot = std::move(t);
+  }
+  return *slot;
+}
 }  // namespace fh
 
 extern "C" int fh_jw_set_method(int method) {
@@ -403,15 +437,8 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     start_a[c + 1] = start_a[c] + ha[c];
     start_b[c + 1] = start_b[c] + hb[c];
   }
-  // Pruning table over the lengths present.
-  std::vector<uint8_t> mmin((kLongLen + 1) * (kLongLen + 1), 255);
-  for (int la = 1; la <= kLongLen; ++la) {
-    if (!ha[la]) continue;
-    for (int lb = 1; lb <= kLongLen; ++lb) {
-      if (!hb[lb]) continue;
-      mmin[la * (kLongLen + 1) + lb] = host_mmin(la, lb, p, lo, hi);
-    }
-  }
+  // Pruning table (every length pair; cached per thresholds).
+  const std::vector<uint8_t>& mmin = mmin_table(p, lo, hi);
   // Plan: waves per A length class, tiles per B length class.
   Plan pl;
   for (int r = 0; r < 3; ++r) pl.wave_lo[r] = pl.wave_hi[r] = 0;

@@ -369,8 +369,7 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
   pl->block_a = block_a;
   const int nkeys_bits = bits_for((uint64_t)n_blocks * kNumClasses);
   const int64_t nk = (int64_t)n_blocks * kNumClasses;
-  std::vector<uint8_t> mmin((kLongLen + 1) * (kLongLen + 1), 255);
-  std::vector<bool> mm_done((kLongLen + 1) * (kLongLen + 1), false);
+  const std::vector<uint8_t>& mmin = mmin_table(p, lo, hi);  // pruning table, cached per thresholds
   std::vector<WorkItem> items[2];
   std::vector<LongItem> long_items;
   uint32_t max_len = 0;
@@ -480,10 +479,6 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
           for (int lb = 1; lb <= kLongLen && ca > 0; ++lb) {
             if (hb[(int64_t)b * kNumClasses + lb] == 0) continue;
             const int idx = c * (kLongLen + 1) + lb;
-            if (!mm_done[idx]) {
-              mmin[idx] = host_mmin(c, lb, p, lo, hi);
-              mm_done[idx] = true;
-            }
             const double pairs = (double)ca * (double)hb[(int64_t)b * kNumClasses + lb];
             if (mmin[idx] <= std::min(c, lb)) {
               pl->pairs_kernel += pairs;

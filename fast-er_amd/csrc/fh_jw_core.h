@@ -226,6 +226,11 @@ inline uint8_t host_mmin(int l1, int l2, float p, float lo, float hi) {
   return 255;
 }
 
+// host_mmin for every length pair 1..kLongLen (entry la * (kLongLen + 1) +
+// lb), computed once per (p, lo, hi) and kept for the process (the table
+// depends on nothing else; computing it takes milliseconds of host time).
+const std::vector<uint8_t>& mmin_table(float p, float lo, float hi);
+
 
 // Launches k_jw_tiles over waves [w0, w0 + nw) of one string-1 length range
 // (r0: 1..16 bytes, r1: 17..32, r2: 33..64) against tiles [t0, t0 + nt) of

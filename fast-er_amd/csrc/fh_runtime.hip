@@ -1,4 +1,6 @@
 // Error reporting, version and fill kernels shared by every entry point.
+#include <mutex>
+
 #include "fh_common.h"
 
 namespace fh {
@@ -13,6 +15,20 @@ void set_error(const char* fmt, ...) {
 }
 
 void clear_error() { g_err[0] = 0; }
+
+void keep_pool_memory() {
+  static std::mutex mu;
+  static bool done[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+  std::lock_guard<std::mutex> lock(mu);
+  if (done[dev]) return;
+  done[dev] = true;
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess || pool == nullptr) return;
+  uint64_t keep = kPoolKeepBytes;
+  (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+}
 
 template <typename T>
 __global__ void k_fill(T* __restrict__ p, T value, int64_t count) {

@@ -74,6 +74,7 @@ _SIGS = {
     "fh_exact_dedup": (_i32, [_vp, _i64, _vp, _vp, _i64, _vp, _i64, ctypes.POINTER(_i64), _vp]),
     "fh_sort_u64": (_i32, [_vp, _i64, _i32, _i32, _vp]),
     "fh_sort_u64_into": (_i32, [_vp, _vp, _i64, _i32, _i32, _vp]),
+    "fh_group_strings_multi": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "fh_group_strings": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                 _vp]),
     "fh_group_values": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, ctypes.POINTER(_i64), _vp]),

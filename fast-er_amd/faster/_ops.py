@@ -40,7 +40,24 @@ def bits_for(max_value: int) -> int:
     return max(1, int(max_value).bit_length())
 
 
-_hit_rate = [0.0]  # largest hits per value pair seen so far (sizes the next arena)
+class _Rates:
+    """Output sizes per input unit seen by recent calls, to size the next
+    call's output buffer. Only calls of a comparable size (at least 1/64 of
+    the new call's units) count, so a tiny call with an extreme rate (every
+    pair a hit at thresholds of 0) does not size a large one."""
+
+    def __init__(self):
+        self.seen = []  # (units, rate), most recent last
+
+    def predict(self, units):
+        r = [rate for u, rate in self.seen if u * 64 >= units]
+        return max(r) if r else 0.0
+
+    def add(self, units, rate):
+        self.seen = (self.seen + [(units, rate)])[-32:]
+
+
+_hit_rate = _Rates()  # hits per value pair (sizes the comparison's hit arena)
 
 
 def jw_levels(chars_a, off_a, n_a, chars_b, off_b, n_b, mode, p, lower_thr, upper_thr,
@@ -53,7 +70,7 @@ def jw_levels(chars_a, off_a, n_a, chars_b, off_b, n_b, mode, p, lower_thr, uppe
     L = _lib.load()
     pairs = float(n_a) * float(n_b)
     if capacity is None:
-        capacity = max(1 << 20, 4 * (int(n_a) + int(n_b)), int(1.25 * _hit_rate[0] * pairs))
+        capacity = max(1 << 20, 4 * (int(n_a) + int(n_b)), int(1.25 * _hit_rate.predict(pairs) * pairs))
     stats = _lib.JwStats()
 
     def run(cap):
@@ -67,7 +84,7 @@ def jw_levels(chars_a, off_a, n_a, chars_b, off_b, n_b, mode, p, lower_thr, uppe
 
     need, hits = _retry(run, capacity)
     if pairs > 0:
-        _hit_rate[0] = max(_hit_rate[0], need / pairs)
+        _hit_rate.add(pairs, need / pairs)
     return hits[:need], (stats.as_dict() if want_stats else None)
 
 
@@ -105,12 +122,12 @@ def expand_link_sorted(hits, n_ua, n_ub, starts_a, rows_a, n_a, starts_b, rows_b
                                       ctypes.byref(n), stream()), "fh_expand_link_sorted")
         return n.value, out
 
-    need, out = _retry(run, max(2 * n_hits, 1024, int(1.25 * _expand_rate[0] * n_hits)))
-    _expand_rate[0] = max(_expand_rate[0], need / n_hits)
+    need, out = _retry(run, max(2 * n_hits, 1024, int(1.25 * _expand_rate.predict(n_hits) * n_hits)))
+    _expand_rate.add(n_hits, need / n_hits)
     return out[:need]
 
 
-_expand_rate = [0.0]  # largest record pairs per hit seen so far (sizes the next output)
+_expand_rate = _Rates()  # record pairs per hit (sizes the sorted expansion's output)
 
 
 def expand_dedup(hits, n_u, starts, rows, n, diag_level):
@@ -194,6 +211,30 @@ def group_strings(chars: torch.Tensor, offsets: torch.Tensor, n: int):
                              ctypes.byref(nu), ctypes.byref(ub), stream()), "fh_group_strings")
     k = nu.value
     return uchars[:max(ub.value, 1)], uoffsets[:k + 1], starts[:k + 1], rows[:n], k
+
+
+def group_strings_multi(cols):
+    """fh_group_strings_multi over [(chars, offsets, n), ...] (up to 16
+    columns): one (unique chars, unique offsets, starts, rows, n_unique) per
+    column, as group_strings gives it."""
+    L = _lib.load()
+    dev = _lib.device()
+    k = len(cols)
+    outs = []
+    for chars, _, n in cols:
+        outs.append((torch.empty(max(int(chars.numel()), 1), dtype=torch.uint8, device=dev), _empty_i64(n + 1),
+                     _empty_i64(n + 1), _empty_i64(n)))
+    P = ctypes.c_void_p * k
+    I = ctypes.c_int64 * k
+    nu, ub = I(), I()
+    check(L.fh_group_strings_multi(k, P(*[ptr(c[0]) for c in cols]), P(*[ptr(c[1]) for c in cols]),
+                                   I(*[int(c[2]) for c in cols]), P(*[ptr(o[0]) for o in outs]),
+                                   P(*[ptr(o[1]) for o in outs]), P(*[ptr(o[2]) for o in outs]),
+                                   P(*[ptr(o[3]) for o in outs]), nu, ub, stream()), "fh_group_strings_multi")
+    res = []
+    for (uchars, uoff, starts, rows), (_, _, n), u, b in zip(outs, cols, nu, ub):
+        res.append((uchars[:max(b, 1)], uoff[:u + 1], starts[:u + 1], rows[:n], u))
+    return res
 
 
 FH_VALUES_I64 = 0

@@ -20,7 +20,7 @@ import torch
 
 from . import _lib, _ops, concurrent, missing, patterns
 from .encode import (DeviceExactLinkGroups, exact_column, group_exact_link, group_exact_link_device, group_packed,
-                     group_strings_device, pack_strings, packed_column_device)
+                     group_packed_multi, group_strings_device, pack_strings, packed_column_device)
 from .levels import level_of_score
 
 _LEVELS_FUZZY = 3
@@ -270,10 +270,13 @@ class Comparison():
         n_a, n_b = inputs.n_a, inputs.n_b
         stats = []
         keys = []
+        # Every fuzzy column of both sides grouped by value in one device sort.
+        flat = group_packed_multi([c for pair in inputs.fuzzy for c in pair]) if inputs.fuzzy else []
+        grouped = [(flat[2 * v], flat[2 * v + 1]) for v in range(len(inputs.fuzzy))]
+
         def fuzzy_task(v):
             def task():
-                ca, cb = inputs.fuzzy[v]
-                ga, gb = group_packed(ca), group_packed(cb)
+                ga, gb = grouped[v]
                 if inputs.missing:
                     ra, rb = inputs.fuzzy_rows[v]
                     ga, gb = _remap_rows(ga, ra), _remap_rows(gb, rb)
@@ -282,20 +285,31 @@ class Comparison():
                 return fuzzy_keys_link(ga, gb, p, Lower_Thr, Upper_Thr, st), st
             return task
 
-        for k, st in concurrent.run([fuzzy_task(v) for v in range(len(inputs.fuzzy))]):
+        if (inputs.numeric or inputs.missing) and Exact_Mode == "rowid":
+            raise Exception("Exact_Mode='rowid' cannot be combined with numeric variables or missing levels.")
+
+        def exact_task():
+            # The exact variables (grouping, pair counts and, in list mode,
+            # their lists) beside the fuzzy chains: they need nothing of them.
+            if inputs.missing:
+                ex = [_exact_groups_link(c, r[0], r[1], r[2], n_a, n_b) for c, r in zip(inputs.exact, inputs.exact_rows)]
+            else:
+                ex = [group_exact_link_device(c, n_a) for c in inputs.exact]
+            tot = [_ops.exact_link_total(g.group_a, g.starts_b, g.rows_b, g.n_groups, n_b) for g in ex]
+            if inputs.numeric or inputs.missing:
+                md = "list" if ex else patterns.choose_exact_mode(Exact_Mode, len(inputs.fuzzy), tot)
+            else:
+                md = patterns.choose_exact_mode(Exact_Mode, len(inputs.fuzzy), tot)
+            lists = [_ops.exact_link(g.group_a, g.starts_b, g.rows_b, g.n_groups, n_b, capacity=t)
+                     for g, t in zip(ex, tot)] if md == "list" else []
+            return None, ex, tot, md, lists
+
+        results = concurrent.run([fuzzy_task(v) for v in range(len(inputs.fuzzy))] + [exact_task])
+        for k, st in results[:-1]:
             keys.append(k)
             stats.extend(st)
-        if inputs.missing:
-            exact = [_exact_groups_link(c, r[0], r[1], r[2], n_a, n_b) for c, r in zip(inputs.exact, inputs.exact_rows)]
-        else:
-            exact = [group_exact_link_device(c, n_a) for c in inputs.exact]
-        totals = [_ops.exact_link_total(g.group_a, g.starts_b, g.rows_b, g.n_groups, n_b) for g in exact]
-        if inputs.numeric or inputs.missing:
-            if Exact_Mode == "rowid":
-                raise Exception("Exact_Mode='rowid' cannot be combined with numeric variables or missing levels.")
-            mode = "list" if exact else patterns.choose_exact_mode(Exact_Mode, len(inputs.fuzzy), totals)
-        else:
-            mode = patterns.choose_exact_mode(Exact_Mode, len(inputs.fuzzy), totals)
+        _, exact, totals, mode, exact_lists = results[-1]
+        concurrent.record_on_current(exact, exact_lists)
         self.Exact_Mode = mode
         self.Stats = stats
         lv_f = [_LEVELS_FUZZY] * len(inputs.fuzzy)
@@ -329,8 +343,7 @@ class Comparison():
             self._rowid_exact = (exact, totals)
             self._Fit_flag = True
             return self
-        for g, t in zip(exact, totals):
-            keys.append(_ops.exact_link(g.group_a, g.starts_b, g.rows_b, g.n_groups, n_b, capacity=t))
+        keys.extend(exact_lists)
         d2, d1 = float(Numeric_Dist[0]), float(Numeric_Dist[1])
         for xa, xb in inputs.numeric:
             keys.append(_ops.numeric_link(xa, xb, d2, d1))

@@ -58,3 +58,27 @@ def run(tasks):
         if isinstance(t, torch.Tensor) and t.is_cuda:
             t.record_stream(main)
     return results
+
+
+def record_on_current(*objs):
+    """Marks every device tensor inside ``objs`` (tensors, lists, tuples,
+    dataclass instances) as used on the current stream, for tensors that a
+    side stream of run() allocated and the caller's stream keeps using."""
+    import dataclasses
+
+    main = torch.cuda.current_stream(_lib.device())
+
+    def walk(o):
+        if isinstance(o, torch.Tensor):
+            if o.is_cuda:
+                o.record_stream(main)
+        elif isinstance(o, (list, tuple)):
+            for x in o:
+                walk(x)
+        elif dataclasses.is_dataclass(o) and not isinstance(o, type):
+            for f in dataclasses.fields(o):
+                walk(getattr(o, f.name))
+
+    for o in objs:
+        walk(o)
+

@@ -19,7 +19,7 @@ import torch
 from . import _lib, _ops, concurrent, missing, patterns
 from .comparison import _split
 from .encode import (DeviceExactDedupGroups, exact_column, group_exact_dedup, group_exact_dedup_device, group_packed,
-                     group_strings_device, packed_column_device)
+                     group_packed_multi, group_strings_device, packed_column_device)
 from .levels import level_of_score
 
 _LEVELS_FUZZY = 3
@@ -153,9 +153,11 @@ class Deduplication():
         n = inputs.n
         stats = []
         keys = []
+        grouped = group_packed_multi(list(inputs.fuzzy)) if inputs.fuzzy else []
+
         def fuzzy_task(k):
             def task():
-                g = group_packed(inputs.fuzzy[k])
+                g = grouped[k]
                 if inputs.missing:
                     g.rows = inputs.fuzzy_rows[k][g.rows]
                     g.n_rows = n

@@ -126,6 +126,21 @@ def group_packed(col: PackedColumn) -> DeviceStringGroups:
     return DeviceStringGroups(n_rows=col.n, n_unique=nu, chars=uchars, offsets=uoff, starts=starts, rows=rows)
 
 
+def group_packed_multi(cols) -> list:
+    """group_packed for several device-resident packed columns in one device
+    sort (fh_group_strings_multi, up to 16 columns per call)."""
+    from . import _ops
+
+    out = []
+    for i in range(0, len(cols), 16):
+        part = cols[i:i + 16]
+        for col, (uchars, uoff, starts, rows, nu) in zip(part, _ops.group_strings_multi(
+                [(c.chars, c.offsets, c.n) for c in part])):
+            out.append(DeviceStringGroups(n_rows=col.n, n_unique=nu, chars=uchars, offsets=uoff, starts=starts,
+                                          rows=rows))
+    return out
+
+
 def group_strings_device(values) -> DeviceStringGroups:
     """Unique values of a string column with their rows (ascending), grouped
     on the device (fh_group_strings): the host only packs the bytes."""

@@ -213,6 +213,19 @@ int fh_group_strings(const uint8_t* chars, const int64_t* offsets, int64_t n, ui
                      int64_t* unique_bytes, fh_stream_t stream);
 
 /*
+ * fh_group_strings for k string columns (1..16) in one device sort: column
+ * c has n[c] rows in chars[c]/offsets[c] and its outputs in uchars[c],
+ * uoffsets[c], starts[c], rows[c] (sized as for fh_group_strings), with the
+ * same contents as k separate calls; n_unique[c] and unique_bytes[c] (host).
+ * The rows of all columns are sorted together by (column, value), so the
+ * columns share the sort's launches and host round trips (two).
+ * Synchronizes the stream.
+ */
+int fh_group_strings_multi(int k, const uint8_t* const* chars, const int64_t* const* offsets, const int64_t* n,
+                           uint8_t* const* uchars, int64_t* const* uoffsets, int64_t* const* starts,
+                           int64_t* const* rows, int64_t* n_unique, int64_t* unique_bytes, fh_stream_t stream);
+
+/*
  * Groups the n rows of a numeric column by value on the device.
  *
  * Replaces the host np.unique(..., return_inverse=True) of an exact column

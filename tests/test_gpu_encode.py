@@ -136,3 +136,32 @@ def test_group_exact_columns_vs_np_unique(case):
     np.testing.assert_array_equal(d.group.cpu().numpy(), inv_a)
     np.testing.assert_array_equal(d.starts.cpu().numpy(), starts_a)
     np.testing.assert_array_equal(d.rows.cpu().numpy(), rows_a)
+
+
+def test_group_strings_multi_equals_single_columns(oracle):
+    """fh_group_strings_multi over columns of different lengths and value
+    sets (empty columns first, in the middle and last; identical values; NUL
+    bytes; values over 64 bytes) gives each column exactly the oracle's
+    grouping (unique values, starts, rows)."""
+    from faster.encode import group_packed_multi, packed_column_device
+
+    rng = np.random.default_rng(21)
+    pool = ["".join("abcd"[k] for k in rng.integers(0, 4, L)) for L in range(0, 90)]
+    cols = [[],
+            [pool[i] for i in rng.integers(0, 30, 4000)],
+            ["same"] * 700,
+            [],
+            ["ab", "ab\x00", "a", "", "zz" * 40, "zz" * 40 + "a"] * 9,
+            [pool[i] for i in rng.integers(0, len(pool), 2500)],
+            []]
+    groups = group_packed_multi([packed_column_device(c) for c in cols])
+    assert len(groups) == len(cols)
+    for k, (values, g) in enumerate(zip(cols, groups)):
+        offs = g.offsets.cpu().numpy()
+        chars = g.chars.cpu().numpy().tobytes()
+        uniq = [chars[offs[i]:offs[i + 1]] for i in range(g.n_unique)]
+        ru, rs, rr = oracle.group_strings(values)
+        assert g.n_rows == len(values) and g.n_unique == len(ru), k
+        assert uniq == ru, k
+        np.testing.assert_array_equal(g.starts.cpu().numpy(), rs, err_msg=f"column {k}")
+        np.testing.assert_array_equal(g.rows.cpu().numpy(), rr, err_msg=f"column {k}")

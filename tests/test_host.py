@@ -198,3 +198,21 @@ def test_pack_column_matches_pack_strings():
         np.testing.assert_array_equal(o1, o2)
         np.testing.assert_array_equal(c1, c2)
         assert o1.dtype == np.int64 and c1.dtype == np.uint8
+
+
+def test_output_size_memory_ignores_much_smaller_calls():
+    """The hit arena and expansion buffers are sized from earlier calls of a
+    comparable size only: a 3,000-pair call where every pair is a hit (rate
+    1) must not size a 1e12-pair call's arena at 1e12 entries."""
+    from faster._ops import _Rates
+
+    r = _Rates()
+    r.add(3000, 1.0)
+    assert r.predict(10 ** 12) == 0.0
+    r.add(10 ** 12, 4.5e-6)
+    assert r.predict(10 ** 12) == 4.5e-6
+    assert r.predict(2 * 10 ** 12) == 4.5e-6  # within 64x
+    assert r.predict(3000) == 1.0  # small calls still use everything seen at or above their size / 64
+    for k in range(40):
+        r.add(10 ** 6, 1e-3)
+    assert len(r.seen) == 32 and r.predict(10 ** 6) == 1e-3
