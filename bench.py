@@ -38,23 +38,13 @@ def log(msg):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-INT8_MFMA_PEAK_OPS = 5.0e15  # dense int8 MFMA, MI355X
-
-
 def comparison_roofline(stats_all, kernel_name, traffic):
     """Roofline of the comparison kernel from the per-call work accounting:
-    int8 MFMA ops of the bound (bound-filtered form) or algorithmic VALU
-    lane-ops (bit-parallel form), over the HIP-event time of the launches."""
+    algorithmic VALU lane-ops over the HIP-event time of the launches."""
     import numpy as np
 
     kern_ms = float(np.mean([sum(s["pass_ms"] for s in st) for st in stats_all]))
-    mfma = float(np.mean([sum(s["alg_mfma_ops"] for s in st) for st in stats_all]))
     lane = float(np.mean([sum(s["alg_lane_ops"] for s in st) for st in stats_all]))
-    if mfma > 0:
-        achieved = mfma / (kern_ms / 1e3)
-        return {"bound": "mfma", "achieved": achieved / 1e12, "peak": INT8_MFMA_PEAK_OPS / 1e12,
-                "unit": "TOP/s (int8)", "frac": achieved / INT8_MFMA_PEAK_OPS, "traffic": traffic,
-                "kernel": kernel_name, "kernel_ms": kern_ms, "alg_ops": mfma}
     # The prefix form skips the recurrence steps of shared prefixes: its
     # algorithmic work is the per-pair count less those steps.
     skipped = float(np.mean([sum(s.get("prefix_skipped_ops", 0.0) for s in st) for st in stats_all]))
@@ -76,8 +66,7 @@ def comparison_roofline(stats_all, kernel_name, traffic):
                              "frac": frozen / (kern_ms / 1e3) / FROZEN_PEAK}}
 
 
-KERNEL_NAMES = {"bound": "k_jw_bound (int8 MFMA bound + exact candidates)",
-                "tiles": "k_jw_tiles (bit-parallel, one string-1 value per lane)",
+KERNEL_NAMES = {"tiles": "k_jw_tiles (bit-parallel, one string-1 value per lane)",
                 "prefix": "k_jw_prefix (bit-parallel, string-1 values walked per wave, shared prefixes reused)"}
 
 
@@ -161,10 +150,10 @@ def main():
     ap.add_argument("--valu-probe", action="store_true",
                     help="run only the VALU throughput probe of the match recurrence (checks the VALU peak)")
     ap.add_argument("--n", type=int, default=None, help="records per side (default 1M link, 2M dedup)")
-    ap.add_argument("--method", choices=["prefix", "tiles", "bound"], default="prefix",
+    ap.add_argument("--method", choices=["prefix", "tiles"], default="prefix",
                     help="all-pairs comparison: the bit-parallel match with string-1 values walked per wave and "
-                         "shared prefixes reused (prefix, default), the same match with one string-1 value per lane "
-                         "(tiles), or an int8 MFMA bound with exact scoring of the candidates (bound)")
+                         "shared prefixes reused (prefix, default), or the same match with one string-1 value per lane "
+                         "(tiles)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-blocked", action="store_true",

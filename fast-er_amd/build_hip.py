@@ -19,7 +19,7 @@ OUT = os.path.join(HERE, "faster", "libfaster_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["fh_jw_blocked.hip", "fh_jw_r1.hip", "fh_jw_r0.hip", "fh_jw_p1.hip", "fh_jw_p0.hip", "fh_jw_p2.hip", "fh_jw_r2.hip",
-           "fh_jw.hip", "fh_jw_bound.hip",
+           "fh_jw.hip",
            "fh_runtime.hip",
            "fh_scan_sort.hip", "fh_pairs.hip", "fh_em.hip", "fh_fslik.hip", "fh_encode.hip"]
 

@@ -128,7 +128,7 @@ const std::vector<uint8_t>& mmin_table(float p, float lo, float hi) {
 
 extern "C" int fh_jw_set_method(int method) {
   fh::clear_error();
-  FH_REQUIRE(method == FH_JW_METHOD_TILES || method == FH_JW_METHOD_BOUND || method == FH_JW_METHOD_PREFIX,
+  FH_REQUIRE(method == FH_JW_METHOD_TILES || method == FH_JW_METHOD_PREFIX,
              "fh_jw_set_method: bad method %d", method);
   fh::g_jw_method.store(method);
   return FH_OK;
@@ -472,23 +472,10 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     }
     pl.tile_hi[k] = (int64_t)pl.tiles.size();
   }
-  // Bound-filtered form: string-2 tiles of kBTile strings of one length.
   const int method = g_jw_method.load();
-  const bool bound = method == FH_JW_METHOD_BOUND;
   // The prefix form needs 256-dword tiles (alphabets of up to about 55 symbols).
   const bool prefix = method == FH_JW_METHOD_PREFIX && tw == kPrefixTileWidth;
   const size_t lds_prefix = jw_prefix_lds_bytes(krows);
-  std::vector<TileMeta> btiles;
-  if (bound) {
-    for (int c = 1; c <= kLongLen; ++c)
-      for (int64_t s0 = start_b[c]; s0 < start_b[c + 1]; s0 += kBTile) {
-        TileMeta tm;
-        tm.len = c;
-        tm.count = (int32_t)std::min<int64_t>(kBTile, start_b[c + 1] - s0);
-        tm.s0 = s0;
-        btiles.push_back(tm);
-      }
-  }
   // Work accounting (host arithmetic over the length histograms).
   if (stats) {
     stats->alphabet = K;
@@ -516,10 +503,8 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       }
     stats->pairs_kernel = (int64_t)kern;
     stats->pairs_pruned = (int64_t)pruned;
-    stats->alg_lane_ops = bound ? 0.0 : ops;
+    stats->alg_lane_ops = ops;
     stats->frozen_model_ops = frozen;
-    // One 64-byte dot product (64 multiply-adds) per pair.
-    stats->alg_mfma_ops = bound ? kern * 128.0 : 0.0;
     stats->method = method;
   }
 
@@ -539,28 +524,15 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   uint32_t* d_table = nullptr;
   const int64_t ntiles = (int64_t)pl.tiles.size();
   FH_TRY(scr.alloc(&d_table, (size_t)std::max<int64_t>(1, ntiles) * krows * tw));
-  TileMeta* d_btiles = nullptr;
-  uint64_t* d_cand = nullptr;  // candidate pairs of one round (bound and prefix forms)
+  uint64_t* d_cand = nullptr;  // candidate pairs of one round (prefix form)
   CandLease cand_lease;        // the prefix form's buffer, kept between calls
   unsigned long long* cand_count = nullptr;
   int64_t cand_cap = (int64_t)1 << 27;
   int64_t* d_cls = nullptr;
-  uint8_t *d_planes_a = nullptr, *d_planes_b = nullptr;
-  const int64_t nbt = (int64_t)btiles.size();
-  if (bound) {
-    FH_TRY(scr.alloc(&d_btiles, std::max<size_t>(1, btiles.size())));
-    FH_TRY(scr.alloc(&d_planes_a, (size_t)std::max<int64_t>(1, n_a) * 64));
-    FH_TRY(scr.alloc(&d_planes_b, (size_t)std::max<int64_t>(1, nbt) * kBTile * 64));
-    if (nbt > 0)
-      FH_HIP(hipMemcpyAsync(d_btiles, btiles.data(), btiles.size() * sizeof(TileMeta), hipMemcpyHostToDevice, s));
-    FH_TRY(jw_bound_build_planes(chars_a, offsets_a, ent_a, n_a, chars_b, offsets_b, ent_b, d_btiles, nbt, d_small, K,
-                                 d_planes_a, d_planes_b, s));
-  }
-  // The bound and prefix forms' candidate pairs go to one buffer per round
-  // and are scored by a separate kernel; the per-lane bit-parallel form
-  // scores its survivors in the kernel. All find a string-2 value's table
-  // column from its sorted index through the class starts and first tile of
-  // each class.
+  // The prefix form's candidate pairs go to one buffer per round and are
+  // scored by a separate kernel; the per-lane form scores its survivors in
+  // the kernel. Both find a string-2 value's table column from its sorted
+  // index through the class starts and first tile of each class.
   //
   // Rounds run over string-1 units (waves), each sized so that its
   // candidates fit the buffer at the candidate rate seen so far; a round that
@@ -570,32 +542,25 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   int64_t unit_lo = 0, unit_hi = (int64_t)pl.waves.size();
   std::vector<double> unit_pairs;  // prefix sums of kernel pairs per unit
   double rate = 4e-3;              // candidates per kernel pair, updated after every round
-  const bool rounds = bound || prefix;
+  const bool rounds = prefix;
   if (rounds) {
     std::vector<double> per_len(kLongLen + 1, 0.0);
-    const int lbmax = prefix ? 32 : kLongLen;  // the prefix form covers string-2 values of up to 32 bytes
     for (int la = 1; la <= kLongLen; ++la)
-      for (int lb = 1; lb <= lbmax; ++lb) {
+      for (int lb = 1; lb <= 32; ++lb) {  // the prefix form covers string-2 values of up to 32 bytes
         if (!hb[lb] || mmin[la * (kLongLen + 1) + lb] > std::min(la, lb)) continue;
         if (!two && lb > la) continue;  // deduplication: string 2 never longer than string 1
         per_len[la] += (!two && lb == la) ? 0.5 * (double)hb[lb] : (double)hb[lb];
       }
-    if (prefix) {
-      unit_lo = pl.wave_lo[0];
-      unit_hi = pl.wave_hi[2];
-    }
+    unit_lo = pl.wave_lo[0];
+    unit_hi = pl.wave_hi[2];
     unit_pairs.assign((size_t)(unit_hi - unit_lo) + 1, 0.0);
     for (int64_t u = unit_lo; u < unit_hi; ++u)
       unit_pairs[u - unit_lo + 1] = unit_pairs[u - unit_lo] + (double)pl.waves[u].count * per_len[pl.waves[u].len];
-    if (prefix) {
-      rate = g_prefix_rate.load();
-      // The first round's budget (0.7 of the buffer at this rate) then covers all units.
-      FH_TRY(cand_lease.acquire((int64_t)std::min(1.5 * rate * unit_pairs.back(), (double)kCandMax), s));
-      d_cand = cand_lease.ptr();
-      cand_cap = cand_lease.cap();
-    } else {
-      FH_TRY(scr.alloc(&d_cand, (size_t)std::max<int64_t>(1, cand_cap)));
-    }
+    rate = g_prefix_rate.load();
+    // The first round's budget (0.7 of the buffer at this rate) then covers all units.
+    FH_TRY(cand_lease.acquire((int64_t)std::min(1.5 * rate * unit_pairs.back(), (double)kCandMax), s));
+    d_cand = cand_lease.ptr();
+    cand_cap = cand_lease.cap();
     FH_TRY(scr.alloc(&cand_count, 1));
   }
   {
@@ -657,21 +622,16 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   args.hmask = 0;
   args.code_stride = 0;
   args.overflow = nullptr;
-  args.n_candidates = bound ? nullptr : counter + 1;  // survivors the bit-parallel pass scores
+  args.n_candidates = counter + 1;  // survivors the per-lane pass scores
   args.cls_start = d_cls;
   args.cls_tile0 = d_cls ? d_cls + kNumClasses : nullptr;
   args.cls_start_a = d_cls ? d_cls + 2 * kNumClasses : nullptr;
-  // The bit-parallel form scores its (rare) survivors inside its VALU-bound
-  // kernel; the bound form hands its candidates to the scoring kernel.
   args.cand = rounds ? d_cand : nullptr;
   args.cand_cap = cand_cap;
   args.cand_count = cand_count;
-  args.planes_a = d_planes_a;
-  args.planes_b = d_planes_b;
   args.codes_a = d_codes;
-  if (bound) args.tiles = d_btiles;
 
-  // d_args: the arguments of the bound and prefix passes and of the scoring
+  // d_args: the arguments of the prefix passes and of the scoring
   // kernel; d_args_lane (no candidate buffer): the per-lane kernel, which
   // scores its survivors itself.
   JwArgs* d_args = nullptr;
@@ -690,7 +650,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   }
   int launches = 0;
   int64_t n_candidates = 0;
-  if (!bound) {
+  {
     // The per-lane kernel: everything, or with the prefix form what it does
     // not cover (string-2 values longer than 32 bytes: kW64 tiles).
     if (stats) FH_HIP(hipEventRecord(evr[0], s));
@@ -730,14 +690,9 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       }
       FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(cand_count), 0, 1, s));
       if (stats) FH_HIP(hipEventRecord(evr[0], s));
-      if (bound) {
-        FH_TRY(launch_jw_bound(mode, d_args, unit_lo + a0, a1 - a0, 0, nbt, s));
-        ++launches;
-      } else {
-        for (int r = 0; r < 3; ++r)
-          FH_TRY(launch_kinds(d_args, lds_bytes, pl, r, unit_lo + a0, unit_lo + a1, s, &launches, kPrefixOnly,
-                              lds_prefix));
-      }
+      for (int r = 0; r < 3; ++r)
+        FH_TRY(launch_kinds(d_args, lds_bytes, pl, r, unit_lo + a0, unit_lo + a1, s, &launches, kPrefixOnly,
+                            lds_prefix));
       if (stats) FH_HIP(hipEventRecord(evr[1], s));
       uint64_t got = 0;
       FH_HIP(hipMemcpyAsync(&got, cand_count, sizeof(got), hipMemcpyDeviceToHost, s));
@@ -750,46 +705,30 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       const double pairs = unit_pairs[a1] - unit_pairs[a0];
       if (pairs > 0) rate = std::max(1e-6, 1.25 * (double)got / pairs);
       if ((int64_t)got > cand_cap) {
-        if (prefix || a1 - a0 == 1) {
-          // A larger buffer and the same round again: the prefix form's
-          // buffer is kept for later calls; a single unit cannot be split.
-          if (prefix) {
-            FH_TRY(cand_lease.grow((int64_t)((double)got * 1.1) + 1, s));
-            FH_REQUIRE(cand_lease.cap() >= (int64_t)got || a1 - a0 > 1,
-                       "fh_jw_levels: %llu candidates in one string-1 unit exceed the buffer",
-                       (unsigned long long)got);
-            if (cand_lease.cap() < (int64_t)got) {
-              const int64_t mid = a0 + (a1 - a0) / 2;
-              todo.push_back({mid, a1});
-              todo.push_back({a0, mid});
-            } else {
-              todo.push_back({a0, a1});
-            }
-            d_cand = cand_lease.ptr();
-            cand_cap = cand_lease.cap();
-          } else {
-            cand_cap = (int64_t)got;
-            FH_TRY(scr.alloc(&d_cand, (size_t)cand_cap));
-            todo.push_back({a0, a1});
-          }
-          args.cand = d_cand;
-          args.cand_cap = cand_cap;
-          FH_HIP(hipMemcpyAsync(d_args, &args, sizeof(JwArgs), hipMemcpyHostToDevice, s));
-          continue;
+        // A larger buffer (kept for later calls) and the same round again,
+        // or the round split in two when the buffer is at its maximum.
+        FH_TRY(cand_lease.grow((int64_t)((double)got * 1.1) + 1, s));
+        FH_REQUIRE(cand_lease.cap() >= (int64_t)got || a1 - a0 > 1,
+                   "fh_jw_levels: %llu candidates in one string-1 unit exceed the buffer", (unsigned long long)got);
+        if (cand_lease.cap() < (int64_t)got) {
+          const int64_t mid = a0 + (a1 - a0) / 2;
+          todo.push_back({mid, a1});
+          todo.push_back({a0, mid});
+        } else {
+          todo.push_back({a0, a1});
         }
-        const int64_t mid = a0 + (a1 - a0) / 2;
-        todo.push_back({mid, a1});
-        todo.push_back({a0, mid});
+        d_cand = cand_lease.ptr();
+        cand_cap = cand_lease.cap();
+        args.cand = d_cand;
+        args.cand_cap = cand_cap;
+        FH_HIP(hipMemcpyAsync(d_args, &args, sizeof(JwArgs), hipMemcpyHostToDevice, s));
         continue;
       }
       prefix_pairs += pairs;
       prefix_got += (double)got;
       n_candidates += (int64_t)got;
       if (stats) FH_HIP(hipEventRecord(evr[2], s));
-      if (bound)
-        FH_TRY(launch_eval_candidates(d_args, (int64_t)got, s));
-      else
-        FH_TRY(launch_eval_prefix(d_args, (int64_t)got, s));
+      FH_TRY(launch_eval_prefix(d_args, (int64_t)got, s));
       ++launches;
       if (stats) {
         FH_HIP(hipEventRecord(evr[3], s));
@@ -799,7 +738,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
         stats->eval_ms += ms;
       }
     }
-    if (prefix && prefix_pairs > 0)
+    if (prefix_pairs > 0)
       g_prefix_rate.store(std::max({1e-6, prefix_got / prefix_pairs, 0.95 * g_prefix_rate.load()}));
   }
 
@@ -885,7 +824,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     }
     stats->prefix_skipped_ops = skipped_ops;
   }
-  if (stats) stats->pairs_candidates = bound ? n_candidates : n_candidates + (int64_t)h_count[1];
+  if (stats) stats->pairs_candidates = n_candidates + (int64_t)h_count[1];
   if (stats) {
     float ms = 0;
     FH_HIP(hipEventElapsedTime(&ms, ev0, ev1));

@@ -590,8 +590,6 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
   FH_TRY(pl->alloc(&pl->n_union, 1));
   for (auto& a : pl->args) {
     a.hkeys = pl->hkeys;
-    a.planes_a = nullptr;
-    a.planes_b = nullptr;
     a.n_candidates = nullptr;
     a.cand = nullptr;
     a.cls_start = nullptr;  // tiles are per block: survivors are scored byte-wise

@@ -75,7 +75,6 @@ constexpr int kLongLen = 64;          // longest string on the bit-parallel path
 constexpr int kMaxGenericLen = 1024;  // longest string scored with its flags in registers
 constexpr int kClassLong = kLongLen + 1;
 constexpr int kNumClasses = kLongLen + 2;  // 0 = empty, 1..64, 65 = long
-constexpr int kBTile = 256;                // string-2 strings per tile of the bound-filtered form
 
 // String-2 mask layouts ("kinds"), chosen by string-2 length. The packed
 // kinds hold L strings per dword in lanes of S bits: lane l's positions
@@ -176,17 +175,14 @@ struct JwArgs {
   uint64_t hmask;
   uint32_t code_stride;
   unsigned int* overflow;
-  // Bound-filtered form (fh_jw_bound.hip): per-string count vectors.
-  const uint8_t* planes_a;  // [sorted index][64]
-  const uint8_t* planes_b;  // [tile][4][kBTile][16]
   unsigned long long* n_candidates;  // pairs scored exactly (may be null)
   const int64_t* cls_start;          // [kNumClasses] first string-2 sorted index of each length
   const int64_t* cls_tile0;          // [kNumClasses] first bit-parallel tile of each length
-  uint64_t* cand;                    // candidate buffer of the bound pass
+  uint64_t* cand;                    // candidate buffer of the prefix form's rounds
   int64_t cand_cap;
   unsigned long long* cand_count;
   // Prefix form (fh_jw_prefix.h): row codes of every string-1 value of at
-  // most 32 bytes, [sorted index][8 dwords], four codes per dword, and the
+  // most 64 bytes, [sorted index][16 dwords], four codes per dword, and the
   // first string-1 sorted index of each length class.
   const uint32_t* codes_a;
   const int64_t* cls_start_a;  // [kNumClasses]
@@ -241,18 +237,6 @@ int launch_jw_tiles_r1(int kind, const JwArgs* args, size_t lds_bytes, int64_t w
                        hipStream_t s);
 int launch_jw_tiles_r2(int kind, const JwArgs* args, size_t lds_bytes, int64_t w0, int64_t nw, int64_t t0, int64_t nt,
                        hipStream_t s);
-
-// Bound-filtered comparison (fh_jw_bound.hip): LDS bytes of one workgroup,
-// the per-string count vectors of both sides, and the launch over string-1
-// units [u0, u0 + nu) x string-2 tiles [t0, t0 + nt) of kBTile strings.
-size_t jw_bound_lds_bytes();
-int jw_bound_build_planes(const uint8_t* chars_a, const int64_t* off_a, const uint64_t* sorted_a, int64_t n_a,
-                          const uint8_t* chars_b, const int64_t* off_b, const uint64_t* sorted_b,
-                          const TileMeta* tiles, int64_t ntiles, const uint8_t* amap, int K, uint8_t* planes_a,
-                          uint8_t* planes_b, hipStream_t s);
-int launch_jw_bound(int mode, const JwArgs* args, int64_t u0, int64_t nu, int64_t t0, int64_t nt, hipStream_t s);
-// Exact scoring of the first n entries of args->cand.
-int launch_eval_candidates(const JwArgs* args, int64_t n, hipStream_t s);
 
 // Device code: internal to every translation unit that includes this header.
 namespace {
@@ -495,12 +479,9 @@ __device__ __forceinline__ uint64_t row_mask(const uint32_t* base, int tw, int c
 }
 
 //
-// kSurvivor: the pair already passed the greedy count (the comparison
-// kernel's survivors), so in deduplication the string order is settled first
-// and a swapped pair goes to the byte-wise restatement without the table's
-// greedy pass; the bound form's candidates (false) take the cheap count test
-// first, as most of them fail it.
-template <bool kSurvivor>
+// The pair already passed the greedy count (a comparison kernel's survivor),
+// so in deduplication the string order is settled first and a swapped pair
+// goes to the byte-wise restatement without the table's greedy pass.
 __device__ __forceinline__ int candidate_level(const JwArgs& a, int64_t sa, int64_t sb, uint64_t& x, uint64_t& y) {
   const uint64_t ea = a.sorted_a[sa], eb = a.sorted_b[sb];
   x = ea;
@@ -509,7 +490,7 @@ __device__ __forceinline__ int candidate_level(const JwArgs& a, int64_t sa, int6
   const int la = (int)(a.off_a[ea + 1] - oa);
   const int64_t ob = a.off_b[eb];
   const int lb = (int)(a.off_b[eb + 1] - ob);
-  if (kSurvivor && a.mode == FH_MODE_DEDUP && !lex_greater(a.chars_a + oa, la, a.chars_b + ob, lb)) {
+  if (a.mode == FH_MODE_DEDUP && !lex_greater(a.chars_a + oa, la, a.chars_b + ob, lb)) {
     const int code = pair_level(a, ea, eb);
     x = eb;
     y = ea;
@@ -544,12 +525,6 @@ __device__ __forceinline__ int candidate_level(const JwArgs& a, int64_t sa, int6
   }
   const int m = __popcll(used);
   if (m < mm) return 0;
-  if (!kSurvivor && a.mode == FH_MODE_DEDUP && !lex_greater(s1, la, a.chars_b + ob, lb)) {
-    const int code = pair_level(a, ea, eb);
-    x = eb;
-    y = ea;
-    return code - kSwapped;
-  }
   float s = 0.0f;
   if (m > 0) {
     int tt = 0;
@@ -618,7 +593,7 @@ __device__ __noinline__ void flush_queue(const JwArgs& a, const uint64_t* q, int
     uint64_t x = 0, y = 0;
     if (k < qn) {
       const uint64_t v = q[k];
-      lvl = candidate_level<true>(a, (int64_t)(s0a + (v & 63ull)), (int64_t)(v >> 6), x, y);
+      lvl = candidate_level(a, (int64_t)(s0a + (v & 63ull)), (int64_t)(v >> 6), x, y);
     }
     emit_levels_wave(a, x, y, lvl);
   }

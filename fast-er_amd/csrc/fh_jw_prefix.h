@@ -512,7 +512,7 @@ __device__ __forceinline__ int prefix_level(const JwArgs& a, const int64_t* star
   const int lb = class_of(starts_b, sb);
   // String-1 values over 32 bytes: the table scorer (its own string order
   // test for deduplication).
-  if (la > 32) return candidate_level<true>(a, sa, sb, x, y);
+  if (la > 32) return candidate_level(a, sa, sb, x, y);
   if (a.mode == FH_MODE_DEDUP) {
     const int64_t oa = a.off_a[ea], ob = a.off_b[eb];
     if (!lex_greater(a.chars_a + oa, la, a.chars_b + ob, lb)) {

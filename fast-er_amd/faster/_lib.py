@@ -42,7 +42,6 @@ class JwStats(ctypes.Structure):
         ("alphabet", ctypes.c_int32),
         ("launches", ctypes.c_int32),
         ("kernel_ms", ctypes.c_double),
-        ("alg_mfma_ops", ctypes.c_double),
         ("method", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
         ("pairs_candidates", ctypes.c_int64),
@@ -174,16 +173,16 @@ def device() -> torch.device:
 
 
 FH_JW_METHOD_TILES = 0
-FH_JW_METHOD_BOUND = 1
 FH_JW_METHOD_PREFIX = 2
-_METHODS = {"tiles": FH_JW_METHOD_TILES, "bound": FH_JW_METHOD_BOUND, "prefix": FH_JW_METHOD_PREFIX}
+_METHODS = {"tiles": FH_JW_METHOD_TILES, "prefix": FH_JW_METHOD_PREFIX}
 
 
 def set_jw_method(name: str) -> None:
     """Selects the all-pairs comparison algorithm of every later call
-    (process-wide): "tiles" (bit-parallel greedy match of every pair; the
-    default) or "bound" (int8 MFMA bound, then exact scoring of the
-    candidates). Both give identical results."""
+    (process-wide): "prefix" (the default: bit-parallel greedy match with
+    each wave walking its sorted string-1 values against a staged string-2
+    tile and reusing shared prefixes) or "tiles" (the same match with one
+    string-1 value per lane). Both give identical results."""
     if name not in _METHODS:
         raise ValueError(f"unknown comparison method {name!r}; expected one of {sorted(_METHODS)}")
     check(load().fh_jw_set_method(_METHODS[name]), "fh_jw_set_method")

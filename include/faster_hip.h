@@ -72,10 +72,9 @@ typedef struct {
   int32_t alphabet;        /* byte values present on both sides */
   int32_t launches;        /* kernel launches of the comparison stage */
   double kernel_ms;        /* device time of the comparison kernels (HIP events) */
-  double alg_mfma_ops;     /* bound-filtered form: int8 MFMA ops of the bound (128 per kernel pair) */
   int32_t method;          /* FH_JW_METHOD_* used */
   int32_t reserved;
-  int64_t pairs_candidates; /* pairs handed to the scoring kernel (bound passed / greedy count passed) */
+  int64_t pairs_candidates; /* pairs handed to the scoring kernel (greedy count passed) */
   double pass_ms;          /* device time of the all-pairs pass kernels alone (HIP events) */
   double eval_ms;          /* device time of the candidate scoring kernels (HIP events) */
   double frozen_model_ops; /* sum over the evaluated pairs of 8 * len1 + 24: a fixed work model of a
@@ -86,13 +85,10 @@ typedef struct {
 } fh_jw_stats;
 
 /*
- * Comparison algorithm of fh_jw_levels (process-wide; all give identical
+ * Comparison algorithm of fh_jw_levels (process-wide; both give identical
  * results):
  *   FH_JW_METHOD_TILES: the bit-parallel greedy match of every pair (VALU),
  *     one string-1 value per lane.
- *   FH_JW_METHOD_BOUND: an int8 MFMA pass bounds every pair's match count by
- *     its symbol-count overlap; pairs whose bound reaches the pruning
- *     threshold are scored exactly.
  *   FH_JW_METHOD_PREFIX (default): the same bit-parallel match with the
  *     string-1 values walked one after another by each wave, whose lanes
  *     cover the string-2 tile; the match state of a prefix shared with the
@@ -103,7 +99,6 @@ typedef struct {
  * forms against each other.
  */
 #define FH_JW_METHOD_TILES 0
-#define FH_JW_METHOD_BOUND 1
 #define FH_JW_METHOD_PREFIX 2
 int fh_jw_set_method(int method);
 int fh_jw_get_method(void);
