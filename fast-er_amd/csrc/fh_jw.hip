@@ -322,11 +322,12 @@ static int launch_kinds(const JwArgs* args, size_t lds, const Plan& pl, int r, i
 // Row codes of the sorted string-1 values [s_begin, s_end) (all of at most
 // 64 bytes) for the prefix form.
 static int launch_prefix_codes(const uint8_t* chars, const int64_t* off, const uint64_t* sorted, int64_t s_begin,
-                               int64_t s_end, const uint8_t* amap, int krows, uint32_t* codes, hipStream_t s) {
+                               int64_t s_end, const uint8_t* amap, int krows, int64_t hi0, int64_t hi_base,
+                               uint32_t* codes, hipStream_t s) {
   if (s_end <= s_begin) return FH_OK;
-  const int64_t blocks = std::min<int64_t>(ceil_div((s_end - s_begin) * kPrefixCodeWords, 256), 8192);
+  const int64_t blocks = std::min<int64_t>(ceil_div((s_end - s_begin) * 16, 256), 8192);
   hipLaunchKernelGGL(k_prefix_codes, dim3((unsigned)blocks), dim3(256), 0, s, chars, off, sorted, s_begin, s_end, amap,
-                     krows, codes);
+                     krows, hi0, hi_base, codes);
   FH_HIP(hipGetLastError());
   return FH_OK;
 }
@@ -575,12 +576,16 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   }
   FH_TRY(launch_build_table(chars_b, offsets_b, ent_b, d_tiles, ntiles, d_small, krows, tw, d_table, s));
   uint32_t* d_codes = nullptr;
+  int64_t codes_hi0 = 0, codes_hi_base = 0;
   unsigned long long* d_skipped = nullptr;  // per string-1 length: steps skipped per tile pass (stats)
   if (prefix) {
     // Values of 1..64 bytes occupy sorted indices [start_a[1], start_a[65]).
     const int64_t c1 = start_a[1], c2 = start_a[kPrefixMaxLen + 1];
-    FH_TRY(scr.alloc(&d_codes, (size_t)std::max<int64_t>(1, n_a) * kPrefixCodeWords));
-    FH_TRY(launch_prefix_codes(chars_a, offsets_a, ent_a, c1, c2, d_small, krows, d_codes, s));
+    const int64_t hi0 = start_a[33], hi_base = std::max<int64_t>(1, n_a) * kPrefixCodeWords;
+    FH_TRY(scr.alloc(&d_codes, (size_t)(hi_base + std::max<int64_t>(0, c2 - hi0) * kPrefixCodeWords)));
+    FH_TRY(launch_prefix_codes(chars_a, offsets_a, ent_a, c1, c2, d_small, krows, hi0, hi_base, d_codes, s));
+    codes_hi0 = hi0;
+    codes_hi_base = hi_base;
     if (stats && c2 > c1) {
       FH_TRY(scr.alloc(&d_skipped, kPrefixMaxLen + 1));
       FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(d_skipped), 0, kPrefixMaxLen + 1, s));
@@ -630,6 +635,8 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   args.cand_cap = cand_cap;
   args.cand_count = cand_count;
   args.codes_a = d_codes;
+  args.codes_hi0 = codes_hi0;
+  args.codes_hi_base = codes_hi_base;
 
   // d_args: the arguments of the prefix passes and of the scoring
   // kernel; d_args_lane (no candidate buffer): the per-lane kernel, which

@@ -182,9 +182,11 @@ struct JwArgs {
   int64_t cand_cap;
   unsigned long long* cand_count;
   // Prefix form (fh_jw_prefix.h): row codes of every string-1 value of at
-  // most 64 bytes, [sorted index][16 dwords], four codes per dword, and the
+  // most 64 bytes, four codes per dword (layout at kPrefixCodeWords), and the
   // first string-1 sorted index of each length class.
   const uint32_t* codes_a;
+  int64_t codes_hi0;      // first sorted index of a value over 32 bytes
+  int64_t codes_hi_base;  // words 8-15 of those values start at codes_a + codes_hi_base
   const int64_t* cls_start_a;  // [kNumClasses]
 };
 

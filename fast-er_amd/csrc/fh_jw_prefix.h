@@ -49,7 +49,11 @@ int launch_eval_prefix(const JwArgs* args, int64_t n, hipStream_t s);
 
 constexpr int kPrefixTileWidth = 256;  // dwords: 64 lanes x 4
 constexpr int kPrefixMaxLen = 64;      // longest string-1 value of the prefix form
-constexpr int kPrefixCodeWords = 16;   // row-code dwords per string-1 value (4 codes each)
+// Row codes of the string-1 values, four per dword: words 0-7 (characters
+// 0-31) of every value at codes_a[sorted index * 8], then words 8-15 of the
+// values over 32 bytes at codes_a[n_a * 8 + (sorted index - codes_hi0) * 8],
+// so that the common values keep a 32-byte stride.
+constexpr int kPrefixCodeWords = 8;
 // LDS bytes of one workgroup of the prefix form: the tile (rows of 256
 // dwords, no padding: every lane reads the same row) and the survivor queues.
 inline size_t jw_prefix_lds_bytes(int krows) {
@@ -301,6 +305,7 @@ __device__ __forceinline__ void prefix_pass_long(const JwArgs& a, PassCtx& c, in
 #pragma unroll 1
   for (int s = 0; s < cntA; ++s) {
     const uint32_t* cp = a.codes_a + (size_t)(c.s0a + s) * kPrefixCodeWords;
+    const uint32_t* cph = a.codes_a + a.codes_hi_base + (size_t)(c.s0a + s - a.codes_hi0) * kPrefixCodeWords - 8;
     const uint32_t w0 = __builtin_amdgcn_readfirstlane(cp[0]);
     int j0 = 0;
     if (s > 0) {
@@ -319,7 +324,7 @@ __device__ __forceinline__ void prefix_pass_long(const JwArgs& a, PassCtx& c, in
     for (int u = 0; u < 4; ++u) na[u] = st[DE - 1][u];
 #pragma unroll 1
     for (int i = DE; i < LA; ++i) {
-      const uint32_t wd = __builtin_amdgcn_readfirstlane(cp[i >> 2]);
+      const uint32_t wd = __builtin_amdgcn_readfirstlane(i < 32 ? cp[i >> 2] : cph[i >> 2]);
       prefix_step<ONES>(prefix_row(lane_addr, (wd >> (8 * (i & 3))) & 0xFFu), nwin_rt<KIND>(i, w, lenB), na, na);
     }
     uint32_t mx = 0;
@@ -436,11 +441,12 @@ __global__ void __launch_bounds__(256) k_prefix_codes(const uint8_t* __restrict_
                                                       const int64_t* __restrict__ off,
                                                       const uint64_t* __restrict__ sorted, int64_t s_begin,
                                                       int64_t s_end, const uint8_t* __restrict__ amap, int krows,
-                                                      uint32_t* __restrict__ codes) {
-  const int64_t n = (s_end - s_begin) * kPrefixCodeWords;
+                                                      int64_t hi0, int64_t hi_base, uint32_t* __restrict__ codes) {
+  const int64_t n = (s_end - s_begin) * 16;
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t si = s_begin + g / kPrefixCodeWords;
-    const int q = (int)(g % kPrefixCodeWords);
+    const int64_t si = s_begin + g / 16;
+    const int q = (int)(g % 16);
+    if (q >= kPrefixCodeWords && si < hi0) continue;  // values of up to 32 bytes: words 0-7 only
     const int64_t e = (int64_t)sorted[si];
     const int64_t o = off[e];
     const int len = (int)(off[e + 1] - o);
@@ -451,7 +457,10 @@ __global__ void __launch_bounds__(256) k_prefix_codes(const uint8_t* __restrict_
       const uint32_t code = i < len ? (uint32_t)amap[chars[o + i]] : (uint32_t)(krows - 1);
       w |= code << (8 * k);
     }
-    codes[si * kPrefixCodeWords + q] = w;
+    if (q < kPrefixCodeWords)
+      codes[si * kPrefixCodeWords + q] = w;
+    else
+      codes[hi_base + (si - hi0) * kPrefixCodeWords + (q - kPrefixCodeWords)] = w;
   }
 }
 
