@@ -400,6 +400,7 @@ __global__ void __launch_bounds__(kThreads) k_member_mask(const int64_t* __restr
 constexpr int kMergeTile = 1024;
 constexpr int kMergeLds = 2048;
 constexpr int kMergeInPlace = 4 * kMergeLds;  // longer ranges are searched in place
+constexpr int kMergeBatch = kMergeTile / kThreads;  // tile entries per thread
 constexpr int kMaxVars = 16;
 
 struct MergeArgs {
@@ -431,6 +432,24 @@ __device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* a, int64_t lo
   return lo;
 }
 
+// First index p[k] in [0, n) with a[p[k]] >= x[k] (a ascending, in LDS), or
+// n, for K keys at once: the K searches take every halving together, so
+// their LDS reads overlap instead of running one search after another.
+template <int K>
+__device__ __forceinline__ void lds_lower_bound(const uint64_t* a, int n, const uint64_t (&x)[K], int (&p)[K]) {
+  int base[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) base[k] = 0;
+  for (int cnt = n; cnt > 1;) {
+    const int half = cnt >> 1;
+#pragma unroll
+    for (int k = 0; k < K; ++k) base[k] = a[base[k] + half] < x[k] ? base[k] + half : base[k];
+    cnt -= half;
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) p[k] = n > 0 && a[base[k]] < x[k] ? base[k] + 1 : base[k];
+}
+
 // bounds[w * (ntiles + 1) + t]: the first entry of list w whose pair id is at
 // least the id of the first entry of tile t of list v, and list w's size for
 // t = ntiles. Consecutive bounds split every other list into one range per
@@ -453,119 +472,233 @@ __global__ void __launch_bounds__(kThreads) k_tile_bounds(MergeArgs m, int v, in
 
 // With `tile_hist`, each tile also counts its owners per code:
 // tile_hist[(code - code_lo) * gridDim.x + tile], for k_code_scatter.
+//
+// The kernel is bound by the latency of its dependent global reads, so the
+// tile's keys and every range's bounds are read in one round trip, all
+// entries of the short ranges a thread looks up are loaded before the first
+// lookup, and the in-place searches of a thread's four tile entries advance
+// in lockstep (one round trip per halving for all four).
 __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, const int64_t* __restrict__ bounds,
                                                             uint32_t* __restrict__ codes, uint32_t code_lo,
                                                             int ncodes, uint32_t* __restrict__ tile_hist) {
   static_assert(kThreads == 256, "one owner-count bin per thread");
+  static_assert(kMergeTile == kMergeBatch * kThreads, "kMergeBatch tile entries per thread");
   __shared__ uint64_t seg[kMergeLds];
   __shared__ uint32_t th[256];
   __shared__ uint64_t tids[kMergeTile];   // the tile's pair ids, ascending
   __shared__ uint32_t tcode[kMergeTile];  // pattern code so far of each tile entry
   __shared__ uint8_t tgone[kMergeTile];   // 1: an earlier variable lists the pair
+  __shared__ const uint64_t* rptr[kMaxVars];  // range of list w: rptr[w][0 .. rlen[w])
+  __shared__ int64_t rlen[kMaxVars];          // 0 for the source list
+  __shared__ uint32_t rstride[kMaxVars];
+  __shared__ int rshort[kMaxVars];            // rlen[w] when at most the tile's length, else 0
   const uint64_t* src = m.lists[v];
   const int64_t n = m.sizes[v];
   const int64_t t0 = (int64_t)blockIdx.x * kMergeTile;
   const int64_t t1 = min(t0 + (int64_t)kMergeTile, n);
   const int tn = (int)(t1 - t0);
   const int64_t per = (int64_t)gridDim.x + 1;
-  th[threadIdx.x] = 0;
-  for (int i = threadIdx.x; i < tn; i += kThreads) {
-    const uint64_t key = src[t0 + i];
-    const uint32_t lvl = (uint32_t)(key & 3ull);
-    if (lvl == 0 || lvl >= m.nlev[v]) atomicOr(m.err, 1u);
-    tids[i] = key >> 2;
-    tcode[i] = lvl * m.stride[v];
-    tgone[i] = 0;
+  const int tid = threadIdx.x;
+  const int nv = m.n_vars;
+  th[tid] = 0;
+  if (tid < nv) {
+    const int64_t b0 = bounds[tid * per + blockIdx.x];
+    const int64_t len = tid == v ? 0 : bounds[tid * per + blockIdx.x + 1] - b0;
+    rptr[tid] = m.lists[tid] + b0;
+    rlen[tid] = len > 0 ? len : 0;  // descending bounds only come from unsorted lists
+    rshort[tid] = len > 0 && len <= tn ? (int)len : 0;
+    rstride[tid] = m.stride[tid];
+  }
+  {
+    // All four keys are loaded before any is used: the error flag's atomic
+    // could alias the lists, so loads after it would wait for it.
+    uint64_t key[kMergeBatch];
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) {
+      const int i = k * kThreads + tid;
+      key[k] = i < tn ? src[t0 + i] : 0ull;
+    }
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) {
+      const int i = k * kThreads + tid;
+      if (i >= tn) continue;
+      const uint32_t lvl = (uint32_t)(key[k] & 3ull);
+      bad = bad || lvl == 0 || lvl >= m.nlev[v];
+      tids[i] = key[k] >> 2;
+      tcode[i] = lvl * m.stride[v];
+      tgone[i] = 0;
+    }
+    if (bad) atomicOr(m.err, 1u);
   }
   __syncthreads();
-  // Ranges no longer than the tile: each of their entries looks itself up
-  // among the tile's ids.
-  for (int w = 0; w < m.n_vars; ++w) {
-    if (w == v) continue;
-    const int64_t b0 = bounds[w * per + blockIdx.x], len = bounds[w * per + blockIdx.x + 1] - b0;
-    if (len > tn) continue;
-    const uint64_t* tl = m.lists[w] + b0;
-    const uint32_t stride = m.stride[w];
-    for (int i = threadIdx.x; i < (int)len; i += kThreads) {
-      const uint64_t x = tl[i];
-      const uint64_t xid = x >> 2;
-      const int64_t p = lower_bound_u64(tids, 0, tn, xid);
-      if (p < tn && tids[p] == xid) {
-        if (w < v)
-          tgone[p] = 1;
+  // Ranges no longer than the tile, taken as one sequence in list order:
+  // each of their entries looks itself up among the tile's ids, up to
+  // kMergeBatch entries per thread at a time (all loaded before the lookups).
+  int nshort = 0;
+  for (int w = 0; w < nv; ++w) nshort += rshort[w];
+  for (int e0 = 0; e0 < nshort; e0 += kMergeBatch * kThreads) {
+    uint64_t x[kMergeBatch], xid[kMergeBatch];
+    int xw[kMergeBatch];
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) {
+      int e = e0 + k * kThreads + tid;
+      xw[k] = -1;
+      x[k] = 0;
+      if (e < nshort) {
+        int w = 0;
+        for (; w < nv; ++w) {
+          if (e < rshort[w]) break;
+          e -= rshort[w];
+        }
+        xw[k] = w;
+        x[k] = rptr[w][e];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) xid[k] = x[k] >> 2;
+    int p[kMergeBatch];
+    lds_lower_bound(tids, tn, xid, p);
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) {
+      if (xw[k] >= 0 && p[k] < tn && tids[p[k]] == xid[k]) {
+        if (xw[k] < v)
+          tgone[p[k]] = 1;
         else
-          atomicAdd(&tcode[p], (uint32_t)(x & 3ull) * stride);
+          atomicAdd(&tcode[p[k]], (uint32_t)(x[k] & 3ull) * rstride[xw[k]]);
       }
     }
   }
   // Ranges far longer than the tile: each tile entry binary-searches the
   // range in place, which touches about log2 of its length entries instead
-  // of reading all of it.
-  for (int w = 0; w < m.n_vars; ++w) {
-    if (w == v) continue;
-    const int64_t b0 = bounds[w * per + blockIdx.x], len = bounds[w * per + blockIdx.x + 1] - b0;
+  // of reading all of it. The four searches of a thread share every step.
+  for (int w = 0; w < nv; ++w) {
+    const int64_t len = rlen[w];
     if (len <= kMergeInPlace) continue;
-    const uint64_t* tl = m.lists[w] + b0;
-    const uint32_t stride = m.stride[w];
-    for (int i = threadIdx.x; i < tn; i += kThreads) {
-      const uint64_t xid = tids[i];
-      const int64_t p = lower_bound_u64(tl, 0, len, xid * 4ull);
-      if (p < len) {
-        const uint64_t x = tl[p];
-        if ((x >> 2) == xid) {
-          if (w < v)
-            tgone[i] = 1;
-          else
-            atomicAdd(&tcode[i], (uint32_t)(x & 3ull) * stride);
-        }
+    const uint64_t* tl = rptr[w];
+    const uint32_t stride = rstride[w];
+    int64_t base[kMergeBatch];
+    uint64_t key[kMergeBatch];
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) {
+      const int i = k * kThreads + tid;
+      key[k] = i < tn ? tids[i] * 4ull : ~0ull;
+      base[k] = 0;
+    }
+    // First index with tl[index] >= key, as base + (tl[base] < key) after
+    // the halvings (every read stays below len).
+    for (int64_t cnt = len; cnt > 1;) {
+      const int64_t half = cnt >> 1;
+      uint64_t probe[kMergeBatch];
+#pragma unroll
+      for (int k = 0; k < kMergeBatch; ++k) probe[k] = tl[base[k] + half];
+#pragma unroll
+      for (int k = 0; k < kMergeBatch; ++k) base[k] = probe[k] < key[k] ? base[k] + half : base[k];
+      cnt -= half;
+    }
+    uint64_t at[kMergeBatch], next[kMergeBatch];
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) {
+      at[k] = tl[base[k]];
+      next[k] = base[k] + 1 < len ? tl[base[k] + 1] : ~0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) {
+      const int i = k * kThreads + tid;
+      if (i >= tn) continue;
+      const bool ge = at[k] >= key[k];
+      const uint64_t xv = ge ? at[k] : next[k];
+      if ((ge || base[k] + 1 < len) && (xv >> 2) == tids[i]) {
+        if (w < v)
+          tgone[i] = 1;
+        else
+          atomicAdd(&tcode[i], (uint32_t)(xv & 3ull) * stride);
       }
     }
   }
   // The ranges in between: staged through LDS a chunk at a time; each tile
   // entry whose id lies within the chunk's ids looks itself up in the chunk.
-  for (int w = 0; w < m.n_vars; ++w) {
-    if (w == v) continue;
-    const int64_t b0 = bounds[w * per + blockIdx.x], len = bounds[w * per + blockIdx.x + 1] - b0;
+  for (int w = 0; w < nv; ++w) {
+    const int64_t len = rlen[w];
     if (len <= tn || len > kMergeInPlace) continue;
-    const uint64_t* tl = m.lists[w] + b0;
-    const uint32_t stride = m.stride[w];
+    const uint64_t* tl = rptr[w];
+    const uint32_t stride = rstride[w];
     for (int64_t c0 = 0; c0 < len; c0 += kMergeLds) {
       const int clen = (int)min((int64_t)kMergeLds, len - c0);
       __syncthreads();
-      for (int i = threadIdx.x; i < clen; i += kThreads) seg[i] = tl[c0 + i];
+      {
+        constexpr int kPer = kMergeLds / kThreads;
+        uint64_t x[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+          const int i = k * kThreads + tid;
+          x[k] = i < clen ? tl[c0 + i] : 0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k)
+          if (k * kThreads + tid < clen) seg[k * kThreads + tid] = x[k];
+      }
       __syncthreads();
       const uint64_t lo_id = seg[0] >> 2, hi_id = seg[clen - 1] >> 2;
-      for (int i = threadIdx.x; i < tn; i += kThreads) {
+      uint64_t key[kMergeBatch];
+      int p[kMergeBatch];
+#pragma unroll
+      for (int k = 0; k < kMergeBatch; ++k) {
+        const int i = k * kThreads + tid;
+        key[k] = i < tn ? tids[i] * 4ull : ~0ull;
+      }
+      lds_lower_bound(seg, clen, key, p);
+#pragma unroll
+      for (int k = 0; k < kMergeBatch; ++k) {
+        const int i = k * kThreads + tid;
+        if (i >= tn) continue;
         const uint64_t xid = tids[i];
         if (xid < lo_id || xid > hi_id) continue;
-        const int64_t p = lower_bound_u64(seg, 0, clen, xid * 4ull);
-        if (p < clen && (seg[p] >> 2) == xid) {
+        if (p[k] < clen && (seg[p[k]] >> 2) == xid) {
           if (w < v)
             tgone[i] = 1;
           else
-            atomicAdd(&tcode[i], (uint32_t)(seg[p] & 3ull) * stride);
+            atomicAdd(&tcode[i], (uint32_t)(seg[p[k]] & 3ull) * stride);
         }
       }
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < tn; i += kThreads) {
-    uint32_t code = tcode[i];
-    if (tgone[i]) {
-      code = 0xFFFFFFFFu;
-    } else if (m.n_imp > 0) {
-      const uint64_t id = tids[i];
-      const uint64_t r1 = id / m.id_div, r2 = id % m.id_div;
-      for (int q = 0; q < m.n_imp; ++q)
-        if (m.imp_a[q][r1] == m.imp_b[q][r2]) code += m.imp_stride[q];
+  // Codes out; each wave adds its owners per code to the tile's counts with
+  // one LDS atomic per distinct code (lanes of one code found by eight
+  // ballots), as a list's owners often share a few codes.
+  const uint64_t lt = fhd::lanemask_lt();
+#pragma unroll
+  for (int k = 0; k < kMergeBatch; ++k) {
+    const int i = k * kThreads + tid;
+    uint32_t code = 0xFFFFFFFFu;
+    if (i < tn) {
+      code = tcode[i];
+      if (tgone[i]) {
+        code = 0xFFFFFFFFu;
+      } else if (m.n_imp > 0) {
+        const uint64_t id = tids[i];
+        const uint64_t r1 = id / m.id_div, r2 = id % m.id_div;
+        for (int q = 0; q < m.n_imp; ++q)
+          if (m.imp_a[q][r1] == m.imp_b[q][r2]) code += m.imp_stride[q];
+      }
+      codes[t0 + i] = code;
     }
-    codes[t0 + i] = code;
-    if (tile_hist && code != 0xFFFFFFFFu) {
-      const uint32_t d = code - code_lo;
-      if (d < (uint32_t)ncodes)
-        atomicAdd(&th[d], 1u);
-      else
+    if (tile_hist) {
+      bool valid = code != 0xFFFFFFFFu;
+      const uint32_t d = valid ? code - code_lo : 0u;
+      if (valid && d >= (uint32_t)ncodes) {
         atomicOr(m.err, 1u);  // only reachable through an invalid level key
+        valid = false;
+      }
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+      }
+      if (valid && (peers & lt) == 0) atomicAdd(&th[d], (uint32_t)__popcll(peers));
     }
   }
   if (tile_hist) {
@@ -575,7 +708,7 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
 }
 
 // Owners of one list, ordered by code (stable: list order within a code),
-// written as (code, pair id) at their final positions: the one counting-sort
+// their pair ids written at their final positions: the one counting-sort
 // pass over the digit code - code_lo (fewer than 256 codes) that replaces
 // flagging, compacting and sorting the owners. A tile is the kMergeTile
 // entries of one k_pattern_codes tile, whose counts the scanned tile_hist
@@ -592,7 +725,6 @@ __global__ void __launch_bounds__(kThreads) k_code_scatter(const uint32_t* __res
                                                            const uint32_t* __restrict__ hist_scanned,
                                                            const uint32_t* __restrict__ list_owners,
                                                            const uint64_t* __restrict__ state, int64_t capacity,
-                                                           uint32_t* __restrict__ out_codes,
                                                            int64_t* __restrict__ ids) {
   if ((int64_t)state[0] + (int64_t)*list_owners > capacity) return;
   uint64_t* __restrict__ out_ids = reinterpret_cast<uint64_t*>(ids + state[0]);
@@ -605,17 +737,23 @@ __global__ void __launch_bounds__(kThreads) k_code_scatter(const uint32_t* __res
   __shared__ uint64_t sid[kMergeTile];
   __shared__ uint32_t scode[kMergeTile];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int64_t wbase = (int64_t)blockIdx.x * kMergeTile + (int64_t)wave * kWaveItems;
+  // Codes and keys of the wave's entries are read up front, in one round trip.
+  uint32_t code[kWaveItems / 64], rnk[kWaveItems / 64];
+  uint64_t key[kWaveItems / 64];
+#pragma unroll
+  for (int r = 0; r < kWaveItems / 64; ++r) {
+    const int64_t e = wbase + r * 64 + lane;
+    code[r] = e < n ? codes[e] : 0xFFFFFFFFu;
+    key[r] = e < n ? keys[e] : 0ull;
+  }
   goff[tid] = tid < ncodes ? hist_scanned[(int64_t)tid * gridDim.x + blockIdx.x] : 0u;
 #pragma unroll
   for (int w = 0; w < kThreads / 64; ++w) wcnt[w][tid] = 0;
   __syncthreads();
   const uint64_t lt = fhd::lanemask_lt();
-  const int64_t wbase = (int64_t)blockIdx.x * kMergeTile + (int64_t)wave * kWaveItems;
-  uint32_t code[kWaveItems / 64], rnk[kWaveItems / 64];
 #pragma unroll
   for (int r = 0; r < kWaveItems / 64; ++r) {
-    const int64_t e = wbase + r * 64 + lane;
-    code[r] = e < n ? codes[e] : 0xFFFFFFFFu;
     // Codes outside the variable's range come only from invalid level keys
     // (flagged by the code kernel); they are dropped, not indexed.
     if (code[r] != 0xFFFFFFFFu && code[r] - code_lo >= (uint32_t)ncodes) code[r] = 0xFFFFFFFFu;
@@ -662,7 +800,7 @@ __global__ void __launch_bounds__(kThreads) k_code_scatter(const uint32_t* __res
     if (code[r] != 0xFFFFFFFFu) {
       const uint32_t d = code[r] - code_lo;
       const uint32_t slot = loff[d] + wcnt[wave][d] + rnk[r];
-      sid[slot] = keys[wbase + r * 64 + lane] >> 2;
+      sid[slot] = key[r] >> 2;
       scode[slot] = code[r];
     }
   }
@@ -670,9 +808,7 @@ __global__ void __launch_bounds__(kThreads) k_code_scatter(const uint32_t* __res
   for (int j = tid; j < owners; j += kThreads) {
     const uint32_t c = scode[j];
     const uint32_t d = c - code_lo;
-    const uint32_t pos = goff[d] + ((uint32_t)j - loff[d]);
-    out_codes[pos] = c;
-    out_ids[pos] = sid[j];
+    out_ids[goff[d] + ((uint32_t)j - loff[d])] = sid[j];
   }
 }
 
@@ -714,11 +850,12 @@ __global__ void __launch_bounds__(kThreads) k_code_bounds(const uint32_t* __rest
   offsets[c - code_lo] = lo;
 }
 
-// offsets[c] = state[0] + first index of code c among the list's owners
-// (out_codes, in code order), for c in [code_lo, code_hi]; state[0] alone
-// when the owners did not fit the output.
-__global__ void __launch_bounds__(kThreads) k_code_offsets(const uint32_t* __restrict__ out_codes,
-                                                           const uint32_t* __restrict__ list_owners,
+// offsets[c] = state[0] + first index of code c among the list's owners,
+// for c in [code_lo, code_hi]: the scanned count of code c's first tile
+// (hist_scanned[(c - code_lo) * ntiles]); state[0] alone when the list is
+// empty or its owners did not fit the output.
+__global__ void __launch_bounds__(kThreads) k_code_offsets(const uint32_t* __restrict__ hist_scanned,
+                                                           int64_t ntiles, const uint32_t* __restrict__ list_owners,
                                                            const uint64_t* __restrict__ state, int64_t capacity,
                                                            uint32_t code_lo, uint32_t code_hi,
                                                            int64_t* __restrict__ offsets) {
@@ -726,18 +863,9 @@ __global__ void __launch_bounds__(kThreads) k_code_offsets(const uint32_t* __res
   if (c > (int64_t)code_hi) return;
   const int64_t base = (int64_t)state[0];
   const int64_t n = (int64_t)*list_owners;
-  int64_t lo = 0;
-  if (base + n <= capacity) {
-    int64_t hi = n;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if ((int64_t)out_codes[mid] < c)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-  }
-  offsets[c] = base + lo;
+  int64_t first = 0;
+  if (hist_scanned != nullptr && base + n <= capacity) first = hist_scanned[(c - code_lo) * ntiles];
+  offsets[c] = base + first;
 }
 
 __global__ void k_advance_base(uint64_t* __restrict__ state, const uint32_t* __restrict__ list_owners) {
@@ -1240,7 +1368,6 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
   int64_t *bounds = nullptr, *tile_bounds = nullptr, *d_offsets = nullptr;
   uint32_t* tile_hist = nullptr;  // owners per (code, tile), for the one-pass code sort
   FH_TRY(scr.alloc(&codes, (size_t)std::max<int64_t>(1, nmax)));
-  FH_TRY(scr.alloc(&ocodes, (size_t)std::max<int64_t>(1, nmax)));
   FH_TRY(scr.alloc(&tot, 1));
   FH_TRY(scr.alloc(&err, 1));
   FH_TRY(scr.alloc(&state, 1));
@@ -1250,6 +1377,7 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
   if (any_general) {
     FH_TRY(scr.alloc(&flags, (size_t)std::max<int64_t>(1, nmax)));
     FH_TRY(scr.alloc(&ocodes_tmp, (size_t)std::max<int64_t>(1, nmax)));
+    FH_TRY(scr.alloc(&ocodes, (size_t)std::max<int64_t>(1, nmax)));
     FH_TRY(scr.alloc(&oids_tmp, (size_t)std::max<int64_t>(1, nmax)));
     FH_TRY(scr.alloc(&bounds, (size_t)n_patterns + 1));
   }
@@ -1278,12 +1406,13 @@ static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const
         FH_TRY(exclusive_scan_u32(tile_hist, tile_hist, (int64_t)ncodes * ntiles, tot, s));
         hipLaunchKernelGGL(k_code_scatter, dim3((unsigned)ntiles), dim3(kThreads), 0, s, codes, m.lists[v], n,
                            code_lo, (int)ncodes, (const uint32_t*)tile_hist, (const uint32_t*)tot,
-                           (const uint64_t*)state, capacity, ocodes, ids);
+                           (const uint64_t*)state, capacity, ids);
         FH_HIP(hipGetLastError());
       } else {
         FH_TRY(fill_u32(tot, 0, 1, s));
       }
-      hipLaunchKernelGGL(k_code_offsets, dim3((unsigned)ceil_div(ncodes, kThreads)), dim3(kThreads), 0, s, ocodes,
+      hipLaunchKernelGGL(k_code_offsets, dim3((unsigned)ceil_div(ncodes, kThreads)), dim3(kThreads), 0, s,
+                         n > 0 ? (const uint32_t*)tile_hist : (const uint32_t*)nullptr, ntiles,
                          (const uint32_t*)tot, (const uint64_t*)state, capacity, code_lo, code_hi, d_offsets);
       FH_HIP(hipGetLastError());
     } else {

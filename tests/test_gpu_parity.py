@@ -406,6 +406,58 @@ def test_pattern_merge_capacity_and_bad_keys():
         assert merge(bad, levels, total, total)[0] == 2
 
 
+
+@pytest.mark.parametrize("levels,sizes", [([3, 3, 3, 2], [3_000_000, 20_000, 100_000, 1_500_000]),
+                                          ([3] * 6, [2_000_000, 15_000, 300_000, 60_000, 1_000_000, 5_000])])
+def test_pattern_merge_lists_of_very_different_sizes(levels, sizes):
+    """Lists of 5k to 3M keys over one id range, so that a tile of one list
+    meets ranges of the others shorter than the tile, a few LDS chunks long,
+    and far longer (searched in place); the second case also has variables
+    owning more than 256 codes (the general owner path). Ids and pattern
+    offsets equal the merge's definition computed with NumPy: the pattern
+    code of a pair is the sum over variables of its level times the
+    variable's stride (first variable slowest, level 0 where unlisted), and
+    the output lists the union's ids by code, ascending within a code."""
+    import ctypes
+
+    from faster import _lib
+
+    L = _lib.load()
+    rng = np.random.default_rng(11)
+    dev = torch.device("cuda")
+    id_range = 100_000_000
+    host = []
+    for n, nl in zip(sizes, levels):
+        ids = np.unique(rng.integers(0, id_range, int(n * 1.05)))
+        ids = np.sort(rng.choice(ids, n, replace=False))
+        host.append((ids, rng.integers(1, nl, n)))
+    keys = [torch.as_tensor(ids * 4 + lv, dtype=torch.int64, device=dev) for ids, lv in host]
+    nv = len(keys)
+    total = sum(sizes)
+    n_pat = int(np.prod(levels))
+    ptrs = (ctypes.c_void_p * nv)(*[k.data_ptr() for k in keys])
+    c_sizes = (ctypes.c_int64 * nv)(*sizes)
+    lv = (ctypes.c_int32 * nv)(*levels)
+    out = torch.full((total,), -7, dtype=torch.int64, device=dev)
+    offs = (ctypes.c_int64 * (n_pat + 1))()
+    nu = ctypes.c_int64(-1)
+    st = L.fh_pattern_merge(nv, ptrs, c_sizes, lv, _lib.ptr(out), ctypes.c_int64(total), offs, ctypes.byref(nu),
+                            _lib.stream())
+    torch.cuda.synchronize()
+    assert st == 0, L.fh_last_error()
+    union = np.unique(np.concatenate([ids for ids, _ in host]))
+    code = np.zeros(len(union), dtype=np.int64)
+    stride = 1
+    for v in range(nv - 1, -1, -1):
+        ids, lvl = host[v]
+        code[np.searchsorted(union, ids)] += lvl * stride
+        stride *= levels[v]
+    order = np.lexsort((union, code))
+    assert nu.value == len(union)
+    np.testing.assert_array_equal(out.cpu().numpy()[:len(union)], union[order])
+    expect = np.concatenate([[0], np.cumsum(np.bincount(code, minlength=n_pat))])
+    np.testing.assert_array_equal(np.asarray(list(offs)), expect)
+
 def test_deduplication_fit(oracle):
     import faster
     from faster.synthetic import dedup_table
