@@ -391,15 +391,12 @@ __global__ void __launch_bounds__(kThreads) k_member_mask(const int64_t* __restr
 //
 // Each block handles a tile of consecutive source entries, whose ids it keeps
 // in LDS, and one range of every other list: the lists are split at the ids
-// of the tiles' first entries by a separate search kernel. The shorter side
-// of each match-up searches the longer one: entries of a range no longer
-// than the tile binary-search the tile's ids; a longer range is staged in LDS
-// a chunk at a time and the tile's entries search it, or, when it is many
-// times longer than the tile, searched in place. Codes accumulate in LDS with
-// atomic adds, as several lists' matches land on one entry.
+// of the tiles' first entries by a separate search kernel. Entries of a
+// range of up to kMergeInPlace entries binary-search the tile's ids; the
+// tile's entries binary-search a longer range in place. Codes accumulate in
+// LDS with atomic adds, as several lists' matches land on one entry.
 constexpr int kMergeTile = 1024;
-constexpr int kMergeLds = 2048;
-constexpr int kMergeInPlace = 4 * kMergeLds;  // longer ranges are searched in place
+constexpr int kMergeInPlace = 8192;  // longer ranges are searched in place
 constexpr int kMergeBatch = kMergeTile / kThreads;  // tile entries per thread
 constexpr int kMaxVars = 16;
 
@@ -470,6 +467,112 @@ __global__ void __launch_bounds__(kThreads) k_tile_bounds(MergeArgs m, int v, in
   bounds[g] = b;
 }
 
+// Lookups of the pattern-code kernel. Entries [e0, e0 + K * kThreads) of the
+// sequence of short ranges (list w's range holds rshort[w] entries, in list
+// order) find their pair among the tile's ids; a match marks the tile entry
+// as not owned (an earlier list) or adds the level times the list's stride.
+template <int K>
+__device__ __forceinline__ void short_lookups(int e0, int nshort, int nv, int v, int tn, const int* rshort,
+                                              const uint64_t* const* rptr, const uint32_t* rstride,
+                                              const uint64_t* tids, uint32_t* tcode, uint8_t* tgone) {
+  uint64_t x[K], xid[K];
+  int xw[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    int e = e0 + k * kThreads + (int)threadIdx.x;
+    xw[k] = -1;
+    x[k] = 0;
+    if (e < nshort) {
+      int w = 0;
+      for (; w < nv; ++w) {
+        if (e < rshort[w]) break;
+        e -= rshort[w];
+      }
+      xw[k] = w;
+      x[k] = rptr[w][e];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) xid[k] = x[k] >> 2;
+  int p[K];
+  lds_lower_bound(tids, tn, xid, p);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (xw[k] >= 0 && p[k] < tn && tids[p[k]] == xid[k]) {
+      if (xw[k] < v)
+        tgone[p[k]] = 1;
+      else
+        atomicAdd(&tcode[p[k]], (uint32_t)(x[k] & 3ull) * rstride[xw[k]]);
+    }
+  }
+}
+
+// The thread's tile entries (kMergeBatch of them) binary-search the ranges of
+// the NL lists ws in place, all searches advancing together: one round trip
+// per halving. The first index with range[index] >= key is base + (range[base]
+// < key) after the halvings, and every read stays inside the range.
+template <int NL>
+__device__ __forceinline__ void inplace_lookups(const int (&ws)[NL], int v, int tn, const int64_t* rlen,
+                                                const uint64_t* const* rptr, const uint32_t* rstride,
+                                                const uint64_t* tids, uint32_t* tcode, uint8_t* tgone) {
+  const int tid = (int)threadIdx.x;
+  int64_t len[NL], cnt[NL], base[NL][kMergeBatch];
+  const uint64_t* tl[NL];
+  uint64_t key[kMergeBatch];
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    len[l] = rlen[ws[l]];
+    cnt[l] = len[l];
+    tl[l] = rptr[ws[l]];
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) base[l][k] = 0;
+  }
+#pragma unroll
+  for (int k = 0; k < kMergeBatch; ++k) {
+    const int i = k * kThreads + tid;
+    key[k] = i < tn ? tids[i] * 4ull : ~0ull;
+  }
+  for (bool more = true; more;) {
+    uint64_t probe[NL][kMergeBatch];
+#pragma unroll
+    for (int l = 0; l < NL; ++l)
+#pragma unroll
+      for (int k = 0; k < kMergeBatch; ++k) probe[l][k] = cnt[l] > 1 ? tl[l][base[l][k] + (cnt[l] >> 1)] : 0ull;
+    more = false;
+#pragma unroll
+    for (int l = 0; l < NL; ++l) {
+      if (cnt[l] <= 1) continue;
+      const int64_t half = cnt[l] >> 1;
+#pragma unroll
+      for (int k = 0; k < kMergeBatch; ++k) base[l][k] = probe[l][k] < key[k] ? base[l][k] + half : base[l][k];
+      cnt[l] -= half;
+      more = more || cnt[l] > 1;
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < NL; ++l) {
+    uint64_t at[kMergeBatch], next[kMergeBatch];
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) {
+      at[k] = tl[l][base[l][k]];
+      next[k] = base[l][k] + 1 < len[l] ? tl[l][base[l][k] + 1] : ~0ull;
+    }
+#pragma unroll
+    for (int k = 0; k < kMergeBatch; ++k) {
+      const int i = k * kThreads + tid;
+      if (i >= tn) continue;
+      const bool ge = at[k] >= key[k];
+      const uint64_t xv = ge ? at[k] : next[k];
+      if ((ge || base[l][k] + 1 < len[l]) && (xv >> 2) == tids[i]) {
+        if (ws[l] < v)
+          tgone[i] = 1;
+        else
+          atomicAdd(&tcode[i], (uint32_t)(xv & 3ull) * rstride[ws[l]]);
+      }
+    }
+  }
+}
+
 // With `tile_hist`, each tile also counts its owners per code:
 // tile_hist[(code - code_lo) * gridDim.x + tile], for k_code_scatter.
 //
@@ -483,7 +586,6 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
                                                             int ncodes, uint32_t* __restrict__ tile_hist) {
   static_assert(kThreads == 256, "one owner-count bin per thread");
   static_assert(kMergeTile == kMergeBatch * kThreads, "kMergeBatch tile entries per thread");
-  __shared__ uint64_t seg[kMergeLds];
   __shared__ uint32_t th[256];
   __shared__ uint64_t tids[kMergeTile];   // the tile's pair ids, ascending
   __shared__ uint32_t tcode[kMergeTile];  // pattern code so far of each tile entry
@@ -491,7 +593,7 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
   __shared__ const uint64_t* rptr[kMaxVars];  // range of list w: rptr[w][0 .. rlen[w])
   __shared__ int64_t rlen[kMaxVars];          // 0 for the source list
   __shared__ uint32_t rstride[kMaxVars];
-  __shared__ int rshort[kMaxVars];            // rlen[w] when at most the tile's length, else 0
+  __shared__ int rshort[kMaxVars];            // rlen[w] when it is not searched in place, else 0
   const uint64_t* src = m.lists[v];
   const int64_t n = m.sizes[v];
   const int64_t t0 = (int64_t)blockIdx.x * kMergeTile;
@@ -506,7 +608,7 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
     const int64_t len = tid == v ? 0 : bounds[tid * per + blockIdx.x + 1] - b0;
     rptr[tid] = m.lists[tid] + b0;
     rlen[tid] = len > 0 ? len : 0;  // descending bounds only come from unsorted lists
-    rshort[tid] = len > 0 && len <= tn ? (int)len : 0;
+    rshort[tid] = len > 0 && len <= kMergeInPlace ? (int)len : 0;
     rstride[tid] = m.stride[tid];
   }
   {
@@ -532,136 +634,34 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
     if (bad) atomicOr(m.err, 1u);
   }
   __syncthreads();
-  // Ranges no longer than the tile, taken as one sequence in list order:
-  // each of their entries looks itself up among the tile's ids, up to
-  // kMergeBatch entries per thread at a time (all loaded before the lookups).
+  // Ranges of up to kMergeInPlace entries, taken as one sequence in list
+  // order: each of their entries looks itself up among the tile's ids, eight
+  // (or, for the last 1,024 or fewer, four) per thread at a time, all loaded
+  // before the lookups.
   int nshort = 0;
   for (int w = 0; w < nv; ++w) nshort += rshort[w];
-  for (int e0 = 0; e0 < nshort; e0 += kMergeBatch * kThreads) {
-    uint64_t x[kMergeBatch], xid[kMergeBatch];
-    int xw[kMergeBatch];
-#pragma unroll
-    for (int k = 0; k < kMergeBatch; ++k) {
-      int e = e0 + k * kThreads + tid;
-      xw[k] = -1;
-      x[k] = 0;
-      if (e < nshort) {
-        int w = 0;
-        for (; w < nv; ++w) {
-          if (e < rshort[w]) break;
-          e -= rshort[w];
-        }
-        xw[k] = w;
-        x[k] = rptr[w][e];
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < kMergeBatch; ++k) xid[k] = x[k] >> 2;
-    int p[kMergeBatch];
-    lds_lower_bound(tids, tn, xid, p);
-#pragma unroll
-    for (int k = 0; k < kMergeBatch; ++k) {
-      if (xw[k] >= 0 && p[k] < tn && tids[p[k]] == xid[k]) {
-        if (xw[k] < v)
-          tgone[p[k]] = 1;
-        else
-          atomicAdd(&tcode[p[k]], (uint32_t)(x[k] & 3ull) * rstride[xw[k]]);
-      }
-    }
-  }
-  // Ranges far longer than the tile: each tile entry binary-searches the
-  // range in place, which touches about log2 of its length entries instead
-  // of reading all of it. The four searches of a thread share every step.
+  int e0 = 0;
+  for (; nshort - e0 > kMergeBatch * kThreads; e0 += 2 * kMergeBatch * kThreads)
+    short_lookups<2 * kMergeBatch>(e0, nshort, nv, v, tn, rshort, rptr, rstride, tids, tcode, tgone);
+  if (e0 < nshort) short_lookups<kMergeBatch>(e0, nshort, nv, v, tn, rshort, rptr, rstride, tids, tcode, tgone);
+  // Longer ranges: each tile entry binary-searches them in place, which
+  // touches about log2 of a range's length entries instead of reading all
+  // of it; two lists at a time, so that eight searches per thread share
+  // every round trip.
+  int pending = -1;
   for (int w = 0; w < nv; ++w) {
-    const int64_t len = rlen[w];
-    if (len <= kMergeInPlace) continue;
-    const uint64_t* tl = rptr[w];
-    const uint32_t stride = rstride[w];
-    int64_t base[kMergeBatch];
-    uint64_t key[kMergeBatch];
-#pragma unroll
-    for (int k = 0; k < kMergeBatch; ++k) {
-      const int i = k * kThreads + tid;
-      key[k] = i < tn ? tids[i] * 4ull : ~0ull;
-      base[k] = 0;
+    if (rlen[w] <= kMergeInPlace) continue;
+    if (pending < 0) {
+      pending = w;
+      continue;
     }
-    // First index with tl[index] >= key, as base + (tl[base] < key) after
-    // the halvings (every read stays below len).
-    for (int64_t cnt = len; cnt > 1;) {
-      const int64_t half = cnt >> 1;
-      uint64_t probe[kMergeBatch];
-#pragma unroll
-      for (int k = 0; k < kMergeBatch; ++k) probe[k] = tl[base[k] + half];
-#pragma unroll
-      for (int k = 0; k < kMergeBatch; ++k) base[k] = probe[k] < key[k] ? base[k] + half : base[k];
-      cnt -= half;
-    }
-    uint64_t at[kMergeBatch], next[kMergeBatch];
-#pragma unroll
-    for (int k = 0; k < kMergeBatch; ++k) {
-      at[k] = tl[base[k]];
-      next[k] = base[k] + 1 < len ? tl[base[k] + 1] : ~0ull;
-    }
-#pragma unroll
-    for (int k = 0; k < kMergeBatch; ++k) {
-      const int i = k * kThreads + tid;
-      if (i >= tn) continue;
-      const bool ge = at[k] >= key[k];
-      const uint64_t xv = ge ? at[k] : next[k];
-      if ((ge || base[k] + 1 < len) && (xv >> 2) == tids[i]) {
-        if (w < v)
-          tgone[i] = 1;
-        else
-          atomicAdd(&tcode[i], (uint32_t)(xv & 3ull) * stride);
-      }
-    }
+    const int ws[2] = {pending, w};
+    inplace_lookups<2>(ws, v, tn, rlen, rptr, rstride, tids, tcode, tgone);
+    pending = -1;
   }
-  // The ranges in between: staged through LDS a chunk at a time; each tile
-  // entry whose id lies within the chunk's ids looks itself up in the chunk.
-  for (int w = 0; w < nv; ++w) {
-    const int64_t len = rlen[w];
-    if (len <= tn || len > kMergeInPlace) continue;
-    const uint64_t* tl = rptr[w];
-    const uint32_t stride = rstride[w];
-    for (int64_t c0 = 0; c0 < len; c0 += kMergeLds) {
-      const int clen = (int)min((int64_t)kMergeLds, len - c0);
-      __syncthreads();
-      {
-        constexpr int kPer = kMergeLds / kThreads;
-        uint64_t x[kPer];
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-          const int i = k * kThreads + tid;
-          x[k] = i < clen ? tl[c0 + i] : 0ull;
-        }
-#pragma unroll
-        for (int k = 0; k < kPer; ++k)
-          if (k * kThreads + tid < clen) seg[k * kThreads + tid] = x[k];
-      }
-      __syncthreads();
-      const uint64_t lo_id = seg[0] >> 2, hi_id = seg[clen - 1] >> 2;
-      uint64_t key[kMergeBatch];
-      int p[kMergeBatch];
-#pragma unroll
-      for (int k = 0; k < kMergeBatch; ++k) {
-        const int i = k * kThreads + tid;
-        key[k] = i < tn ? tids[i] * 4ull : ~0ull;
-      }
-      lds_lower_bound(seg, clen, key, p);
-#pragma unroll
-      for (int k = 0; k < kMergeBatch; ++k) {
-        const int i = k * kThreads + tid;
-        if (i >= tn) continue;
-        const uint64_t xid = tids[i];
-        if (xid < lo_id || xid > hi_id) continue;
-        if (p[k] < clen && (seg[p[k]] >> 2) == xid) {
-          if (w < v)
-            tgone[i] = 1;
-          else
-            atomicAdd(&tcode[i], (uint32_t)(seg[p[k]] & 3ull) * stride);
-        }
-      }
-    }
+  if (pending >= 0) {
+    const int ws[1] = {pending};
+    inplace_lookups<1>(ws, v, tn, rlen, rptr, rstride, tids, tcode, tgone);
   }
   __syncthreads();
   // Codes out; each wave adds its owners per code to the tile's counts with
