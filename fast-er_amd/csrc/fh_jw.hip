@@ -648,30 +648,35 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   JwArgs args_lane = args;
   args_lane.cand = nullptr;
   FH_HIP(hipMemcpyAsync(d_args_lane, &args_lane, sizeof(JwArgs), hipMemcpyHostToDevice, s));
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, evr[4] = {nullptr, nullptr, nullptr, nullptr};
-  if (stats) {
-    FH_HIP(hipEventCreate(&ev0));
-    FH_HIP(hipEventCreate(&ev1));
-    for (auto& e : evr) FH_HIP(hipEventCreate(&e));
-    FH_HIP(hipEventRecord(ev0, s));
-  }
+  // Timings for stats: events recorded around the launches and read only
+  // after the final synchronisation, so that asking for stats adds no wait.
+  struct Events {
+    std::vector<hipEvent_t> v;
+    ~Events() {
+      for (hipEvent_t e : v) (void)hipEventDestroy(e);
+    }
+  } events;
+  std::vector<std::pair<std::pair<size_t, size_t>, double*>> spans;  // (start, end) -> accumulator
+  auto mark = [&]() -> size_t {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return SIZE_MAX;
+    events.v.push_back(e);
+    (void)hipEventRecord(e, s);
+    return events.v.size() - 1;
+  };
+  size_t ev0 = SIZE_MAX, ev1 = SIZE_MAX;
+  if (stats) ev0 = mark();
   int launches = 0;
   int64_t n_candidates = 0;
   {
     // The per-lane kernel: everything, or with the prefix form what it does
     // not cover (string-2 values longer than 32 bytes: kW64 tiles).
-    if (stats) FH_HIP(hipEventRecord(evr[0], s));
+    const size_t e0 = stats ? mark() : SIZE_MAX;
     const LaunchSet set = prefix ? kLaneRest : kLaneAll;
     for (int r = 0; r < 3; ++r)
       FH_TRY(launch_kinds(d_args_lane, lds_bytes, pl, r, 0, (int64_t)pl.waves.size(), s, &launches, set,
                           lds_prefix));
-    if (stats) {
-      FH_HIP(hipEventRecord(evr[1], s));
-      FH_HIP(hipEventSynchronize(evr[1]));
-      float ms = 0;
-      FH_HIP(hipEventElapsedTime(&ms, evr[0], evr[1]));
-      stats->pass_ms += ms;
-    }
+    if (stats) spans.push_back({{e0, mark()}, &stats->pass_ms});
   }
   if (rounds) {
     const int64_t nunits = unit_hi - unit_lo;
@@ -696,19 +701,14 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
         u0 = a1;
       }
       FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(cand_count), 0, 1, s));
-      if (stats) FH_HIP(hipEventRecord(evr[0], s));
+      const size_t e0 = stats ? mark() : SIZE_MAX;
       for (int r = 0; r < 3; ++r)
         FH_TRY(launch_kinds(d_args, lds_bytes, pl, r, unit_lo + a0, unit_lo + a1, s, &launches, kPrefixOnly,
                             lds_prefix));
-      if (stats) FH_HIP(hipEventRecord(evr[1], s));
+      if (stats) spans.push_back({{e0, mark()}, &stats->pass_ms});
       uint64_t got = 0;
       FH_HIP(hipMemcpyAsync(&got, cand_count, sizeof(got), hipMemcpyDeviceToHost, s));
       FH_HIP(hipStreamSynchronize(s));
-      if (stats) {
-        float ms = 0;
-        FH_HIP(hipEventElapsedTime(&ms, evr[0], evr[1]));
-        stats->pass_ms += ms;
-      }
       const double pairs = unit_pairs[a1] - unit_pairs[a0];
       if (pairs > 0) rate = std::max(1e-6, 1.25 * (double)got / pairs);
       if ((int64_t)got > cand_cap) {
@@ -734,16 +734,10 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       prefix_pairs += pairs;
       prefix_got += (double)got;
       n_candidates += (int64_t)got;
-      if (stats) FH_HIP(hipEventRecord(evr[2], s));
+      const size_t e2 = stats ? mark() : SIZE_MAX;
       FH_TRY(launch_eval_prefix(d_args, (int64_t)got, s));
       ++launches;
-      if (stats) {
-        FH_HIP(hipEventRecord(evr[3], s));
-        FH_HIP(hipEventSynchronize(evr[3]));
-        float ms = 0;
-        FH_HIP(hipEventElapsedTime(&ms, evr[2], evr[3]));
-        stats->eval_ms += ms;
-      }
+      if (stats) spans.push_back({{e2, mark()}, &stats->eval_ms});
     }
     if (prefix_pairs > 0)
       g_prefix_rate.store(std::max({1e-6, prefix_got / prefix_pairs, 0.95 * g_prefix_rate.load()}));
@@ -803,9 +797,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       if (two) FH_TRY(huge(start_b[kClassLong], nsb_long, n_a, 1));
     }
   }
-  if (stats) {
-    FH_HIP(hipEventRecord(ev1, s));
-  }
+  if (stats) ev1 = mark();
   uint64_t h_count[2] = {0, 0};
   FH_HIP(hipMemcpyAsync(h_count, counter, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   std::vector<unsigned long long> h_skipped(kPrefixMaxLen + 1, 0);
@@ -833,13 +825,16 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   }
   if (stats) stats->pairs_candidates = n_candidates + (int64_t)h_count[1];
   if (stats) {
-    float ms = 0;
-    FH_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-    stats->kernel_ms = ms;
+    // Every event is complete after the synchronisation above.
+    auto elapsed = [&](size_t a, size_t b) -> double {
+      float ms = 0;
+      if (a == SIZE_MAX || b == SIZE_MAX || hipEventElapsedTime(&ms, events.v[a], events.v[b]) != hipSuccess)
+        return 0.0;
+      return (double)ms;
+    };
+    for (const auto& sp : spans) *sp.second += elapsed(sp.first.first, sp.first.second);
+    stats->kernel_ms = elapsed(ev0, ev1);
     stats->launches = launches;
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    for (auto& e : evr) (void)hipEventDestroy(e);
   }
   *n_hits = (int64_t)h_count[0];
   return FH_OK;

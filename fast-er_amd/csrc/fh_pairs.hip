@@ -180,11 +180,15 @@ __global__ void __launch_bounds__(kThreads) k_expand_partners(const uint64_t* __
 
 // Unique value of every A row (rows of value g: rows_a[starts_a[g] ..
 // starts_a[g + 1])).
+// One thread per listed row (starts_a[n_ua] <= n_a of them), which finds its
+// value by a search of the starts: a value shared by many rows does not
+// leave one thread writing all of them.
 __global__ void __launch_bounds__(kThreads) k_row_values(const int64_t* __restrict__ starts_a,
                                                          const int64_t* __restrict__ rows_a, int64_t n_ua,
-                                                         int64_t* __restrict__ value_of) {
-  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_ua; g += (int64_t)gridDim.x * blockDim.x)
-    for (int64_t k = starts_a[g]; k < starts_a[g + 1]; ++k) value_of[rows_a[k]] = g;
+                                                         int64_t n_a, int64_t* __restrict__ value_of) {
+  const int64_t np = min(starts_a[n_ua], n_a);
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < np; p += (int64_t)gridDim.x * blockDim.x)
+    value_of[rows_a[p]] = upper_index(starts_a, n_ua, (uint64_t)p);
 }
 
 // First entry of each value's partner list in the sorted partners
@@ -1165,7 +1169,7 @@ extern "C" int fh_expand_link_sorted(const uint64_t* hits, int64_t n_hits, int64
                              bits_for(((uint64_t)n_ua * (uint64_t)n_b - 1) * 4 + 3), s));
   // Rows without a value in the groups (not listed in rows_a) pair with nothing.
   FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(value_of), ~0ull, n_a, s));
-  hipLaunchKernelGGL(k_row_values, dim3(grid_for(n_ua)), dim3(kThreads), 0, s, starts_a, rows_a, n_ua, value_of);
+  hipLaunchKernelGGL(k_row_values, dim3(grid_for(n_a)), dim3(kThreads), 0, s, starts_a, rows_a, n_ua, n_a, value_of);
   FH_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_partner_bounds, dim3(grid_for(n_ua + 1)), dim3(kThreads), 0, s, part, (int64_t)n_part, n_ua,
                      n_b, first);
