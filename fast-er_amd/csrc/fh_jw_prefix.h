@@ -505,32 +505,15 @@ __global__ void __launch_bounds__(256) k_prefix_restarts(const uint32_t* __restr
     if (h[i]) atomicAdd(&skipped[i], h[i]);
 }
 
-// Level of survivor (sa, sb) of the prefix form, string 1 = sa: the
+// Level of string 1 = the value with row codes cw and length la against
+// string 2 = sorted value sb of length lb (both at most 32 bytes): the
 // reference's greedy match, transpositions and prefix
 // (faster/comparison.py:48-123) from string 2's mask-table column and
 // string 1's row codes, with all of string 1's rows read at once into
 // registers (candidate_level reads them again for the transpositions and
-// the prefix). Deduplication pairs whose string order is reversed go to the
-// byte-wise restatement, as in candidate_level.
-__device__ __forceinline__ int prefix_level(const JwArgs& a, const int64_t* starts_a, const int64_t* starts_b,
-                                            int64_t sa, int64_t sb, uint64_t& x, uint64_t& y) {
-  const uint64_t ea = a.sorted_a[sa], eb = a.sorted_b[sb];
-  x = ea;
-  y = eb;
-  const int la = class_of(starts_a, sa);
-  const int lb = class_of(starts_b, sb);
-  // String-1 values over 32 bytes: the table scorer (its own string order
-  // test for deduplication).
-  if (la > 32) return candidate_level(a, sa, sb, x, y);
-  if (a.mode == FH_MODE_DEDUP) {
-    const int64_t oa = a.off_a[ea], ob = a.off_b[eb];
-    if (!lex_greater(a.chars_a + oa, la, a.chars_b + ob, lb)) {
-      const int code = pair_level(a, ea, eb);
-      x = eb;
-      y = ea;
-      return code - kSwapped;
-    }
-  }
+// the prefix).
+__device__ __forceinline__ int table_level(const JwArgs& a, const int64_t* starts_b, const uint32_t (&cw)[8], int la,
+                                           int64_t sb, int lb) {
   const int mm = a.mmin[la * (kLongLen + 1) + lb];
   const int kind = kind_of_len(lb);
   const int64_t pos = sb - starts_b[lb];
@@ -544,10 +527,6 @@ __device__ __forceinline__ int prefix_level(const JwArgs& a, const int64_t* star
   }
   const uint32_t* base = a.table + (size_t)tile * a.krows * a.tw + col;
   const uint32_t lmask = lb >= 32 ? ~0u : ((1u << lb) - 1u);
-  const uint32_t* cp = a.codes_a + (size_t)sa * kPrefixCodeWords;
-  uint32_t cw[8];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) cw[k] = cp[k];
   // Row i: string 2's positions holding string 1's byte i.
   uint32_t row[32];
 #pragma unroll
@@ -592,14 +571,100 @@ __device__ __forceinline__ int prefix_level(const JwArgs& a, const int64_t* star
   return level_of(jw_formula(la, lb, m, tt, pre, a.p), a.lo, a.hi);
 }
 
+// Row codes of sorted value si (string-1 side).
+__device__ __forceinline__ void load_codes(const JwArgs& a, int64_t si, uint32_t (&cw)[8]) {
+  const uint32_t* cp = a.codes_a + (size_t)si * kPrefixCodeWords;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) cw[k] = cp[k];
+}
+
+// Byte order of two values of at most 32 bytes from their row codes: in
+// deduplication both sides are one set, so every byte present has its own
+// code, assigned in byte order; the first differing position decides, then
+// the length (lex_greater's order).
+__device__ __forceinline__ bool codes_greater(const uint32_t (&c1)[8], int l1, const uint32_t (&c2)[8], int l2) {
+  int d = 32;
+#pragma unroll
+  for (int k = 7; k >= 0; --k) {
+    const uint32_t x = c1[k] ^ c2[k];
+    if (x) d = 4 * k + (int)(__builtin_ctz(x) >> 3);
+  }
+  if (d < min(l1, l2)) {
+    const int k = d >> 2, sh = 8 * (d & 3);
+    uint32_t b1 = 0, b2 = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j == k) {
+        b1 = (c1[j] >> sh) & 0xFFu;
+        b2 = (c2[j] >> sh) & 0xFFu;
+      }
+    return b1 > b2;
+  }
+  return l1 > l2;
+}
+
+// Level of survivor (sa, sb) of the prefix form, string 1 = sa, by
+// table_level; string-1 values over 32 bytes go to candidate_level.
+// Deduplication pairs whose byte order is reversed are scored with the roles
+// exchanged (string 1 = sb): in deduplication both sides share one sorted
+// order, one code array and one mask table, so sb's row codes and sa's
+// table column exist.
+__device__ __forceinline__ int prefix_level(const JwArgs& a, const int64_t* starts_a, const int64_t* starts_b,
+                                            int64_t sa, int64_t sb, uint64_t& x, uint64_t& y) {
+  const uint64_t ea = a.sorted_a[sa], eb = a.sorted_b[sb];
+  x = ea;
+  y = eb;
+  const int la = class_of(starts_a, sa);
+  const int lb = class_of(starts_b, sb);
+  if (la > 32) return candidate_level(a, sa, sb, x, y);
+  uint32_t cw[8];
+  load_codes(a, sa, cw);
+  int l1 = la, l2 = lb;
+  int64_t s2 = sb;
+  if (a.mode == FH_MODE_DEDUP) {
+    uint32_t cw2[8];
+    load_codes(a, sb, cw2);
+    if (!codes_greater(cw, la, cw2, lb)) {
+      x = eb;
+      y = ea;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) cw[k] = cw2[k];
+      l1 = lb;
+      l2 = la;
+      s2 = sa;
+    }
+  }
+  return table_level(a, starts_b, cw, l1, s2, l2);
+}
+
+// Appends the wave's buffered level keys to a.hits with one atomic on the
+// hit counter (entries past a.cap are counted only).
+__device__ __forceinline__ void eval_flush(const JwArgs& a, const uint64_t* buf, int n) {
+  if (n == 0) return;
+  unsigned long long base = 0;
+  if (__lane_id() == 0) base = atomicAdd(a.counter, (unsigned long long)n);
+  base = __shfl(base, 0, 64);
+  for (int k = (int)__lane_id(); k < n; k += 64)
+    if ((int64_t)(base + (unsigned long long)k) < a.cap) a.hits[base + k] = buf[k];
+}
+
+// Level keys leave through a per-wave LDS buffer of kEvalBuf entries, one
+// atomic on the shared hit counter per flush: one returning atomic per wave
+// and iteration on that single address serialised the scorer.
+constexpr int kEvalBuf = 256;
+
 __global__ void __launch_bounds__(256) k_eval_prefix(const JwArgs* __restrict__ ap, int64_t n) {
   const JwArgs& a = *ap;
   __shared__ int64_t starts[2][kNumClasses];
+  __shared__ uint64_t hbuf[256 / 64][kEvalBuf];
   for (int i = threadIdx.x; i < kNumClasses; i += blockDim.x) {
     starts[0][i] = a.cls_start_a[i];
     starts[1][i] = a.cls_start[i];
   }
   __syncthreads();
+  uint64_t* buf = hbuf[threadIdx.x >> 6];
+  int nbuf = 0;  // wave-uniform
+  const uint64_t lt = fhd::lanemask_lt();
   // Wave-uniform trip count, so every lane reaches the per-wave emission.
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t e0 = (int64_t)blockIdx.x * blockDim.x; e0 < n; e0 += stride) {
@@ -610,8 +675,21 @@ __global__ void __launch_bounds__(256) k_eval_prefix(const JwArgs* __restrict__ 
       const uint64_t v = a.cand[e];
       lvl = prefix_level(a, starts[0], starts[1], (int64_t)(v >> 32), (int64_t)(v & 0xFFFFFFFFull), x, y);
     }
-    emit_levels_wave(a, x, y, lvl);
+    if (a.hkeys) {
+      emit_level(a, x, y, lvl);
+      continue;
+    }
+    const uint64_t mask = __ballot(lvl > 0);
+    if (mask == 0) continue;
+    const int cnt = __popcll(mask);
+    if (nbuf + cnt > kEvalBuf) {
+      eval_flush(a, buf, nbuf);
+      nbuf = 0;
+    }
+    if (lvl > 0) buf[nbuf + __popcll(mask & lt)] = ((uint64_t)x * (uint64_t)a.n_b + y) * 4ull + (uint64_t)lvl;
+    nbuf += cnt;
   }
+  eval_flush(a, buf, nbuf);
 }
 
 template <int LMIN, int LMAX, int KIND>
