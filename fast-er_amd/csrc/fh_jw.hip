@@ -458,25 +458,27 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     }
     pl.wave_hi[r] = (int64_t)pl.waves.size();
   }
+  const int method = g_jw_method.load();
+  // The prefix form needs 256-dword tiles (alphabets of up to about 55 symbols).
+  const bool prefix = method == FH_JW_METHOD_PREFIX && tw == kPrefixTileWidth;
+  const size_t lds_prefix = jw_prefix_lds_bytes(krows);
+  // The prefix form's plans put string-2 values of 1..5 bytes in kP5 tiles.
   for (int k = 0; k < kNumKinds; ++k) {
     pl.tile_lo[k] = (int64_t)pl.tiles.size();
     for (int c = 1; c <= kLongLen; ++c) {
-      if (kind_of_len(c) != k) continue;
+      if (kind_for(c, prefix) != k) continue;
       const int per = kind_tile_strings(k, tw);
       for (int64_t s0 = start_b[c]; s0 < start_b[c + 1]; s0 += per) {
         TileMeta tm;
         tm.len = c;
         tm.count = (int32_t)std::min<int64_t>(per, start_b[c + 1] - s0);
         tm.s0 = s0;
+        tm.kind = k;
         pl.tiles.push_back(tm);
       }
     }
     pl.tile_hi[k] = (int64_t)pl.tiles.size();
   }
-  const int method = g_jw_method.load();
-  // The prefix form needs 256-dword tiles (alphabets of up to about 55 symbols).
-  const bool prefix = method == FH_JW_METHOD_PREFIX && tw == kPrefixTileWidth;
-  const size_t lds_prefix = jw_prefix_lds_bytes(krows);
   // Work accounting (host arithmetic over the length histograms).
   if (stats) {
     stats->alphabet = K;
@@ -499,7 +501,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
           continue;
         }
         kern += pairs;
-        ops += pairs * kind_pair_ops(kind_of_len(lb), la);
+        ops += pairs * kind_pair_ops(kind_for(lb, prefix), la);
         frozen += pairs * (8.0 * la + 24.0);
       }
     stats->pairs_kernel = (int64_t)kern;
@@ -614,6 +616,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   args.krows = krows;
   args.tw = tw;
   args.mode = mode;
+  args.p5 = prefix ? 1 : 0;
   args.n_b = n_b;
   args.p = p;
   args.lo = lo;
@@ -815,7 +818,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
       for (int lb = 1; lb <= 32; ++lb) {
         if (!hb[lb] || mmin[la * (kLongLen + 1) + lb] > std::min(la, lb)) continue;
         if (!two && lb > la) continue;
-        const int k = kind_of_len(lb);
+        const int k = kind_for(lb, prefix);
         const double nb = (!two && lb == la) ? 0.5 * (double)hb[lb] : (double)hb[lb];
         per_step += nb * 3.0 / (packed_kind(k) ? kind_lanes(k) : 1);
       }

@@ -474,6 +474,7 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
             tm.len = c;
             tm.count = (int32_t)std::min<int64_t>(per, sb + cb - s0);
             tm.s0 = s0;
+            tm.kind = kind_of_len(c);
             tiles.push_back(tm);
           }
           for (int lb = 1; lb <= kLongLen && ca > 0; ++lb) {
@@ -555,6 +556,7 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
     a.krows = krows;
     a.tw = pl->tw;
     a.mode = FH_MODE_LINK;
+    a.p5 = 0;  // the blocked kernel is the per-lane form: no kP5 tiles
     a.n_b = n_b;
     a.p = p;
     a.lo = lo;

@@ -84,17 +84,21 @@ constexpr int kNumClasses = kLongLen + 2;  // 0 = empty, 1..64, 65 = long
 // into it, and its +1 enters at position 0 itself, which lets three strings
 // of 10 bytes share a dword.
 //   kind  lengths  L  S   guard of lane 0
+//   kP5   1-5      5  6   yes   (prefix form only; kP4 elsewhere)
 //   kP4   1-7      4  8   yes
 //   kP3   8-10     3  11  no
 //   kP2   11-15    2  16  yes
-enum Kind { kP4 = 0, kP3 = 1, kP2 = 2, kW32 = 3, kW64 = 4 };
-constexpr int kNumKinds = 5;
+// The per-lane form keeps one survivor bit per (dword, lane) of an 8-dword
+// step in a 32-bit mask, which five lanes would overflow, so kP5 tiles are
+// planned only for the prefix form (kind_for).
+enum Kind { kP4 = 0, kP3 = 1, kP2 = 2, kW32 = 3, kW64 = 4, kP5 = 5 };
+constexpr int kNumKinds = 6;
 
-__host__ __device__ constexpr bool packed_kind(int k) { return k <= kP2; }
+__host__ __device__ constexpr bool packed_kind(int k) { return k <= kP2 || k == kP5; }
 // Strings per dword of a packed kind, the stride of its lanes and the bit of
 // lane 0's first position.
-__host__ __device__ constexpr int kind_lanes(int k) { return k == kP4 ? 4 : (k == kP3 ? 3 : 2); }
-__host__ __device__ constexpr int kind_lane_bits(int k) { return k == kP4 ? 8 : (k == kP3 ? 11 : 16); }
+__host__ __device__ constexpr int kind_lanes(int k) { return k == kP5 ? 5 : (k == kP4 ? 4 : (k == kP3 ? 3 : 2)); }
+__host__ __device__ constexpr int kind_lane_bits(int k) { return k == kP5 ? 6 : (k == kP4 ? 8 : (k == kP3 ? 11 : 16)); }
 __host__ __device__ constexpr int kind_lane_base(int k) { return k == kP3 ? 0 : 1; }
 // First bit of string position 0 in lane l.
 __host__ __device__ constexpr int kind_lane_shift(int k, int l) { return kind_lane_base(k) + l * kind_lane_bits(k); }
@@ -107,10 +111,14 @@ __host__ __device__ constexpr uint32_t kind_ones(int k) {
 __host__ __device__ constexpr int kind_of_len(int len) {
   return len <= 7 ? kP4 : (len <= 10 ? kP3 : (len <= 15 ? kP2 : (len <= 32 ? kW32 : kW64)));
 }
+// The kind of a string-2 length in a plan that may use kP5 (p5: the prefix
+// form's plans).
+__host__ __device__ constexpr int kind_for(int len, bool p5) { return p5 && len <= 5 ? kP5 : kind_of_len(len); }
 // Every lane's positions fit the dword, and the lanes fill it.
 static_assert(kind_ones(kP4) == 0x01010101u && kind_lane_shift(kP4, 3) + 7 == 32, "kP4 layout");
 static_assert(kind_ones(kP3) == 0x00200401u && kind_lane_shift(kP3, 2) + 10 == 32, "kP3 layout");
 static_assert(kind_ones(kP2) == 0x00010001u && kind_lane_shift(kP2, 1) + 15 == 32, "kP2 layout");
+static_assert(kind_ones(kP5) == 0x01041041u && kind_lane_shift(kP5, 4) + 5 == 30, "kP5 layout");
 // Strings of one tile of `tw` dword columns.
 __host__ __device__ constexpr int kind_tile_strings(int k, int tw) {
   return packed_kind(k) ? kind_lanes(k) * tw : (k == kW32 ? tw : tw / 2);
@@ -144,6 +152,7 @@ struct TileMeta {
   int32_t len;    // string-2 length of every string in the tile
   int32_t count;  // strings in the tile
   int64_t s0;     // first sorted index
+  int32_t kind;   // mask layout of the tile's strings (Kind)
 };
 
 struct JwArgs {
@@ -161,6 +170,7 @@ struct JwArgs {
   int krows;
   int tw;
   int mode;
+  int p5;       // 1: string-2 values of 1..5 bytes are in kP5 tiles (kind_for)
   int64_t n_b;  // id stride (n_b for linkage, n for deduplication)
   float p, lo, hi;
   uint64_t* hits;
@@ -499,7 +509,7 @@ __device__ __forceinline__ int candidate_level(const JwArgs& a, int64_t sa, int6
     return code - kSwapped;
   }
   const int mm = a.mmin[la * (kLongLen + 1) + lb];
-  const int kind = kind_of_len(lb);
+  const int kind = kind_for(lb, a.p5 != 0);
   const int64_t pos = sb - a.cls_start[lb];
   const int per = kind_tile_strings(kind, a.tw);
   const int64_t tile = a.cls_tile0[lb] + pos / per;
@@ -955,7 +965,7 @@ __device__ __forceinline__ void run_item(const JwArgs& a, uint32_t* lds, int64_t
       // Only the columns the passes read: the tile's used columns rounded up
       // to a power of two >= kU (columns past the tile's strings hold stale
       // data, whose candidates the passes drop by index).
-      const int shift = tile_cols_log2(kind_of_len(tm.len), tm.count, tw);
+      const int shift = tile_cols_log2(tm.kind, tm.count, tw);
       const int cols = 1 << shift;
       const uint32_t* src = a.table + (size_t)t * a.krows * tw;
       const int total = a.krows << shift;
@@ -982,7 +992,7 @@ __device__ __forceinline__ void run_item(const JwArgs& a, uint32_t* lds, int64_t
     if (c.mm > min(lenA, c.lenB)) continue;
     c.w = max(lenA, c.lenB) / 2 - 1;
     if constexpr (KIND == kMixed) {
-      const int kind = kind_of_len(c.lenB);
+      const int kind = tm.kind;
       if constexpr (LMAX <= 32) {
         if (kind == kP4)
           PassDispatch<LMIN, LMAX, kP4, LMAX>::run(lenA, a, c, off);
@@ -1046,7 +1056,7 @@ __global__ void __launch_bounds__(256) k_build_table(const uint8_t* __restrict__
     __syncthreads();
     for (int i = threadIdx.x; i < total; i += blockDim.x) tl[i] = 0xFFFFFFFFu;
     __syncthreads();
-    const int kind = kind_of_len(tm.len);
+    const int kind = tm.kind;
     const int n = tm.count * tm.len;
     for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
       const int sidx = idx / tm.len;

@@ -515,7 +515,7 @@ __global__ void __launch_bounds__(256) k_prefix_restarts(const uint32_t* __restr
 __device__ __forceinline__ int table_level(const JwArgs& a, const int64_t* starts_b, const uint32_t (&cw)[8], int la,
                                            int64_t sb, int lb) {
   const int mm = a.mmin[la * (kLongLen + 1) + lb];
-  const int kind = kind_of_len(lb);
+  const int kind = kind_for(lb, a.p5 != 0);
   const int64_t pos = sb - starts_b[lb];
   const int per = kind_tile_strings(kind, a.tw);
   const int64_t tile = a.cls_tile0[lb] + pos / per;
@@ -726,6 +726,7 @@ template <int LMIN, int LMAX>
 int launch_prefix_kind(int kind, const JwArgs* args, size_t lds, int64_t w0, int64_t nw, int64_t t0, int64_t nt,
                        hipStream_t s) {
   switch (kind) {
+    case kP5: return launch_prefix<LMIN, LMAX, kP5>(args, lds, w0, nw, t0, nt, s);
     case kP4: return launch_prefix<LMIN, LMAX, kP4>(args, lds, w0, nw, t0, nt, s);
     case kP3: return launch_prefix<LMIN, LMAX, kP3>(args, lds, w0, nw, t0, nt, s);
     case kP2: return launch_prefix<LMIN, LMAX, kP2>(args, lds, w0, nw, t0, nt, s);
