@@ -164,4 +164,30 @@ __device__ __forceinline__ uint64_t value_key(int64_t v, bool is_f64) {
 // Key of the canonical NaN (the largest float64 key).
 constexpr uint64_t kNanKey = 0xFFF8000000000000ull;
 
+// v reduced over the workgroup by op (blockDim.x a multiple of 64, at most
+// 1024); the result is valid in thread 0. Every thread must call it. Kernels
+// that fold a grid-wide extreme or count into one global word reduce here
+// first and add one atomic per workgroup: one per wave on a single address
+// serialises at the memory side.
+template <typename T, typename Op>
+__device__ __forceinline__ T block_reduce(T v, Op op) {
+  __shared__ T part[16];
+  for (int d = 32; d >= 1; d >>= 1) v = op(v, (T)__shfl_xor(v, d));
+  if (__lane_id() == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) v = op(v, part[i]);
+  __syncthreads();  // part is reused by the next call
+  return v;
+}
+struct OpMax {
+  __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const { return a > b ? a : b; }
+};
+struct OpMin {
+  __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const { return a < b ? a : b; }
+};
+struct OpSum {
+  __device__ unsigned long long operator()(unsigned long long a, unsigned long long b) const { return a + b; }
+};
+
 }  // namespace fhd

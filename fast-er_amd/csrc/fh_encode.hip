@@ -21,22 +21,38 @@ namespace {
 
 constexpr int kEncThreads = 256;
 
+// Strings per word count min(ceil(len / 8), kWordBins - 1), for the suffix
+// sorts of the word passes (see word_suffixes).
+constexpr int kWordBins = 65;
+
+__device__ __forceinline__ void count_words(unsigned int* hw, uint64_t len) {
+  const uint64_t words = (len + 7) / 8;
+  atomicAdd(&hw[words < (uint64_t)(kWordBins - 1) ? words : (uint64_t)(kWordBins - 1)], 1u);
+}
+
+__device__ __forceinline__ void flush_words(const unsigned int* hw, unsigned long long* wcount) {
+  for (int j = threadIdx.x; j < kWordBins; j += blockDim.x)
+    if (hw[j]) atomicAdd(wcount + j, (unsigned long long)hw[j]);
+}
+
 __global__ void __launch_bounds__(kEncThreads) k_enc_init(const int64_t* __restrict__ off, int64_t n,
                                                           uint64_t* __restrict__ perm, uint64_t* __restrict__ lens,
-                                                          unsigned long long* __restrict__ max_len) {
+                                                          unsigned long long* __restrict__ max_len,
+                                                          unsigned long long* __restrict__ wcount) {
+  __shared__ unsigned int hw[kWordBins];
+  for (int j = threadIdx.x; j < kWordBins; j += blockDim.x) hw[j] = 0;
+  __syncthreads();
   unsigned long long m = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t len = (uint64_t)(off[i + 1] - off[i]);
     perm[i] = (uint64_t)i;
     lens[i] = len;
     m = m > len ? m : len;
+    count_words(hw, len);
   }
-  // Wave maximum, then one atomic per wave.
-  for (int d = 32; d >= 1; d >>= 1) {
-    const unsigned long long o = __shfl_xor(m, d);
-    m = m > o ? m : o;
-  }
-  if (__lane_id() == 0) atomicMax(max_len, m);
+  m = fhd::block_reduce(m, fhd::OpMax());  // its barriers also order the LDS counts
+  if (threadIdx.x == 0) atomicMax(max_len, m);
+  flush_words(hw, wcount);
 }
 
 // keys[i] = bytes [8 * word, 8 * word + 8) of row perm[i], zero-padded,
@@ -118,6 +134,27 @@ __global__ void __launch_bounds__(kEncThreads) k_enc_copy(const uint8_t* __restr
   }
 }
 
+// Grid of the kernels that fold one global extreme or count (one atomic per
+// workgroup).
+constexpr unsigned kReduceGrid = 1024;
+
+// Rows whose strings are longer than 8 * w bytes, for w = 0 .. words - 1,
+// from the word counts (wcount[j]: strings of min(ceil(len / 8), 64) words).
+// After the stable sort by length, the strings of at most 8 * w bytes form a
+// prefix of the order, and their word-w key is 0 in every byte: a stable
+// sort of the whole order by word w leaves that prefix first and sorts the
+// rest, so each word pass sorts only the suffix of longer strings (a larger
+// suffix, as for w >= 64, is equally correct).
+std::vector<int64_t> word_suffixes(const unsigned long long* wcount, int words) {
+  std::vector<int64_t> r((size_t)std::max(words, 0), 0);
+  for (int w = 0; w < words; ++w) {
+    int64_t c = 0;
+    for (int j = std::min(w + 1, kWordBins - 1); j < kWordBins; ++j) c += (int64_t)wcount[j];
+    r[w] = c;
+  }
+  return r;
+}
+
 unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kEncThreads), 8192)); }
 
 // ------------------------- numeric value grouping --------------------------
@@ -132,12 +169,9 @@ __global__ void __launch_bounds__(kEncThreads) k_val_keys(const int64_t* __restr
     lo = k < lo ? k : lo;
     hi = k > hi ? k : hi;
   }
-  for (int d = 32; d >= 1; d >>= 1) {
-    const unsigned long long a = __shfl_xor(lo, d), b = __shfl_xor(hi, d);
-    lo = a < lo ? a : lo;
-    hi = b > hi ? b : hi;
-  }
-  if (__lane_id() == 0) {
+  lo = fhd::block_reduce(lo, fhd::OpMin());
+  hi = fhd::block_reduce(hi, fhd::OpMax());
+  if (threadIdx.x == 0) {
     atomicMin(minmax, lo);
     atomicMax(minmax + 1, hi);
   }
@@ -227,7 +261,11 @@ __device__ __forceinline__ int col_of(const ColSet& cs, int64_t g) {
 
 __global__ void __launch_bounds__(kEncThreads) k_encm_init(ColSet cs, uint64_t* __restrict__ perm,
                                                            uint64_t* __restrict__ lens,
-                                                           unsigned long long* __restrict__ max_len) {
+                                                           unsigned long long* __restrict__ max_len,
+                                                           unsigned long long* __restrict__ wcount) {
+  __shared__ unsigned int hw[kWordBins];
+  for (int j = threadIdx.x; j < kWordBins; j += blockDim.x) hw[j] = 0;
+  __syncthreads();
   unsigned long long m = 0;
   const int64_t n = cs.base[cs.k];
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
@@ -237,19 +275,17 @@ __global__ void __launch_bounds__(kEncThreads) k_encm_init(ColSet cs, uint64_t* 
     perm[g] = (uint64_t)g;
     lens[g] = len;
     m = m > len ? m : len;
+    count_words(hw, len);
   }
-  for (int d = 32; d >= 1; d >>= 1) {
-    const unsigned long long o = __shfl_xor(m, d);
-    m = m > o ? m : o;
-  }
-  if (__lane_id() == 0) atomicMax(max_len, m);
+  m = fhd::block_reduce(m, fhd::OpMax());  // its barriers also order the LDS counts
+  if (threadIdx.x == 0) atomicMax(max_len, m);
+  flush_words(hw, wcount);
 }
 
 // keys[i] = bytes [8 * word, 8 * word + 8) of global row perm[i] (zero-padded,
-// first byte most significant), or its column when word < 0.
-__global__ void __launch_bounds__(kEncThreads) k_encm_keys(ColSet cs, const uint64_t* __restrict__ perm, int word,
-                                                           uint64_t* __restrict__ keys) {
-  const int64_t n = cs.base[cs.k];
+// first byte most significant), or its column when word < 0; i in [0, n).
+__global__ void __launch_bounds__(kEncThreads) k_encm_keys(ColSet cs, const uint64_t* __restrict__ perm, int64_t n,
+                                                           int word, uint64_t* __restrict__ keys) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t g = (int64_t)perm[i];
     const int c = col_of(cs, g);
@@ -388,27 +424,35 @@ extern "C" int fh_group_strings(const uint8_t* chars, const int64_t* offsets, in
   FH_TRY(scr.alloc(&perm_tmp, (size_t)n));
   FH_TRY(scr.alloc(&keys, (size_t)n + 1));
   FH_TRY(scr.alloc(&keys_tmp, (size_t)n + 1));
-  FH_TRY(scr.alloc(&small, 4));  // [0] max length, [1] unique count, [2] unique bytes
-  FH_TRY(fill_u64(small, 0, 4, s));
+  // [0] max length, [1] unique count, [2] unique bytes, [4 ..] word counts
+  FH_TRY(scr.alloc(&small, 4 + kWordBins));
+  FH_TRY(fill_u64(small, 0, 4 + kWordBins, s));
   const unsigned grid = grid_for(n);
 
-  // Pass 0: lengths (also the maximum length, which sets the word count).
-  hipLaunchKernelGGL(k_enc_init, dim3(grid), dim3(kEncThreads), 0, s, offsets, n, perm, keys,
-                     reinterpret_cast<unsigned long long*>(small));
+  // Pass 0: lengths (also the maximum length, which sets the word count,
+  // and the strings per word count).
+  hipLaunchKernelGGL(k_enc_init, dim3(std::min(grid, kReduceGrid)), dim3(kEncThreads), 0, s, offsets, n, perm, keys,
+                     reinterpret_cast<unsigned long long*>(small), reinterpret_cast<unsigned long long*>(small + 4));
   FH_HIP(hipGetLastError());
-  uint64_t max_len = 0;
-  FH_HIP(hipMemcpyAsync(&max_len, small, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  std::vector<unsigned long long> h_small(4 + kWordBins);
+  FH_HIP(hipMemcpyAsync(h_small.data(), small, h_small.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   FH_HIP(hipStreamSynchronize(s));
+  const uint64_t max_len = h_small[0];
   FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, n, 0, std::max(1, bits_for(max_len)), s));
 
-  // Words from the last to the first; bytes past max_len are zero in every
-  // key, so the low bits of a partial last word need no pass.
+  // Words from the last to the first, each over the suffix of strings longer
+  // than its first byte (word_suffixes); bytes past max_len are zero in
+  // every key, so the low bits of a partial last word need no pass.
   const int words = (int)ceil_div((int64_t)max_len, 8);
+  const std::vector<int64_t> suffix = word_suffixes(h_small.data() + 4, words);
   for (int w = words - 1; w >= 0; --w) {
-    hipLaunchKernelGGL(k_enc_word_keys, dim3(grid), dim3(kEncThreads), 0, s, chars, offsets, perm, n, w, keys);
+    const int64_t m = suffix[w], s0 = n - m;
+    if (m <= 0) continue;
+    hipLaunchKernelGGL(k_enc_word_keys, dim3(grid_for(m)), dim3(kEncThreads), 0, s, chars, offsets, perm + s0, m, w,
+                       keys + s0);
     FH_HIP(hipGetLastError());
     const int valid_bytes = (int)std::min<int64_t>(8, (int64_t)max_len - 8 * (int64_t)w);
-    FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, n, 8 * (8 - valid_bytes), 64, s));
+    FH_TRY(radix_sort_u64(keys + s0, keys_tmp + s0, perm + s0, perm_tmp + s0, m, 8 * (8 - valid_bytes), 64, s));
   }
 
   // Value boundaries, value ids, starts and packed unique bytes.
@@ -479,27 +523,33 @@ extern "C" int fh_group_strings_multi(int k, const uint8_t* const* chars, const 
   FH_TRY(scr.alloc(&perm_tmp, (size_t)N));
   FH_TRY(scr.alloc(&keys, (size_t)N + 1));
   FH_TRY(scr.alloc(&keys_tmp, (size_t)N + 1));
-  FH_TRY(scr.alloc(&small, 4));  // [0] max length, [1] value count, [2] value bytes
+  // [0] max length, [1] value count, [2] value bytes, [4 ..] word counts
+  FH_TRY(scr.alloc(&small, 4 + kWordBins));
   FH_TRY(scr.alloc(&counts, 2 * (size_t)k));
-  FH_TRY(fill_u64(small, 0, 4, s));
+  FH_TRY(fill_u64(small, 0, 4 + kWordBins, s));
   const unsigned grid = grid_for(N);
-  hipLaunchKernelGGL(k_encm_init, dim3(grid), dim3(kEncThreads), 0, s, cs, perm, keys,
-                     reinterpret_cast<unsigned long long*>(small));
+  hipLaunchKernelGGL(k_encm_init, dim3(std::min(grid, kReduceGrid)), dim3(kEncThreads), 0, s, cs, perm, keys,
+                     reinterpret_cast<unsigned long long*>(small), reinterpret_cast<unsigned long long*>(small + 4));
   FH_HIP(hipGetLastError());
-  uint64_t max_len = 0;
-  FH_HIP(hipMemcpyAsync(&max_len, small, sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+  std::vector<unsigned long long> h_small(4 + kWordBins);
+  FH_HIP(hipMemcpyAsync(h_small.data(), small, h_small.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
   FH_HIP(hipStreamSynchronize(s));
-  // Length, then the words from the last to the first, then the column.
+  const uint64_t max_len = h_small[0];
+  // Length, then the words from the last to the first (each over the suffix
+  // of strings longer than its first byte, word_suffixes), then the column.
   FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, N, 0, std::max(1, bits_for(max_len)), s));
   const int words = (int)ceil_div((int64_t)max_len, 8);
+  const std::vector<int64_t> suffix = word_suffixes(h_small.data() + 4, words);
   for (int w = words - 1; w >= 0; --w) {
-    hipLaunchKernelGGL(k_encm_keys, dim3(grid), dim3(kEncThreads), 0, s, cs, perm, w, keys);
+    const int64_t m = suffix[w], s0 = N - m;
+    if (m <= 0) continue;
+    hipLaunchKernelGGL(k_encm_keys, dim3(grid_for(m)), dim3(kEncThreads), 0, s, cs, perm + s0, m, w, keys + s0);
     FH_HIP(hipGetLastError());
     const int valid_bytes = (int)std::min<int64_t>(8, (int64_t)max_len - 8 * (int64_t)w);
-    FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, N, 8 * (8 - valid_bytes), 64, s));
+    FH_TRY(radix_sort_u64(keys + s0, keys_tmp + s0, perm + s0, perm_tmp + s0, m, 8 * (8 - valid_bytes), 64, s));
   }
   if (k > 1) {
-    hipLaunchKernelGGL(k_encm_keys, dim3(grid), dim3(kEncThreads), 0, s, cs, perm, -1, keys);
+    hipLaunchKernelGGL(k_encm_keys, dim3(grid), dim3(kEncThreads), 0, s, cs, perm, N, -1, keys);
     FH_HIP(hipGetLastError());
     FH_TRY(radix_sort_u64(keys, keys_tmp, perm, perm_tmp, N, 0, bits_for((uint64_t)k - 1), s));
   }
@@ -551,7 +601,7 @@ extern "C" int fh_group_values(const int64_t* values, int64_t n, int kind, int64
   FH_TRY(fill_u64(small, ~0ull, 1, s));
   FH_TRY(fill_u64(small + 1, 0, 3, s));
   const unsigned grid = grid_for(n);
-  hipLaunchKernelGGL(k_val_keys, dim3(grid), dim3(kEncThreads), 0, s, values, n, kind, keys, perm,
+  hipLaunchKernelGGL(k_val_keys, dim3(std::min(grid, kReduceGrid)), dim3(kEncThreads), 0, s, values, n, kind, keys, perm,
                      reinterpret_cast<unsigned long long*>(small));
   FH_HIP(hipGetLastError());
   uint64_t mm[2] = {0, 0};

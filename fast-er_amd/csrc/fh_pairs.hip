@@ -914,13 +914,10 @@ __global__ void __launch_bounds__(kThreads) k_num_keys(const double* __restrict_
     hi = k > hi ? k : hi;
     nan += k == fhd::kNanKey ? 1ull : 0ull;
   }
-  for (int d = 32; d >= 1; d >>= 1) {
-    const unsigned long long a = __shfl_xor(lo, d), b = __shfl_xor(hi, d), c = __shfl_xor(nan, d);
-    lo = a < lo ? a : lo;
-    hi = b > hi ? b : hi;
-    nan += c;
-  }
-  if (__lane_id() == 0) {
+  lo = fhd::block_reduce(lo, fhd::OpMin());
+  hi = fhd::block_reduce(hi, fhd::OpMax());
+  nan = fhd::block_reduce(nan, fhd::OpSum());
+  if (threadIdx.x == 0) {
     atomicMin(small, lo);
     atomicMax(small + 1, hi);
     if (nan) atomicAdd(small + 2, nan);
@@ -1530,7 +1527,7 @@ static int sort_values(const double* v, int64_t n, Scratch& scr, double** vs, ui
   FH_TRY(scr.alloc(&small, 4));
   FH_TRY(fill_u64(small, ~0ull, 1, s));
   FH_TRY(fill_u64(small + 1, 0, 3, s));
-  hipLaunchKernelGGL(k_num_keys, dim3(grid_for(n)), dim3(kThreads), 0, s, v, n, keys, pm,
+  hipLaunchKernelGGL(k_num_keys, dim3(grid_for(n, 1024)), dim3(kThreads), 0, s, v, n, keys, pm,
                      reinterpret_cast<unsigned long long*>(small));
   FH_HIP(hipGetLastError());
   uint64_t h[3] = {0, 0, 0};

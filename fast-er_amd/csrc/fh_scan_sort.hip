@@ -28,11 +28,11 @@ __device__ __forceinline__ T wave_inclusive_scan(T v) {
   return v;
 }
 
-// Exclusive scan of one value per thread across the block; returns the
-// exclusive prefix and writes the block total to *total.
-template <typename T>
+// Exclusive scan of one value per thread across a block of THREADS threads;
+// returns the exclusive prefix and writes the block total to *total.
+template <typename T, int THREADS = kScanThreads>
 __device__ __forceinline__ T block_exclusive_scan(T v, T* total) {
-  __shared__ T wave_tot[kScanThreads / 64];
+  __shared__ T wave_tot[THREADS / 64];
   const int lane = __lane_id();
   const int wave = threadIdx.x / 64;
   T inc = wave_inclusive_scan(v);
@@ -40,7 +40,7 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* total) {
   __syncthreads();
   T base = 0, all = 0;
 #pragma unroll
-  for (int w = 0; w < kScanThreads / 64; ++w) {
+  for (int w = 0; w < THREADS / 64; ++w) {
     T t = wave_tot[w];
     if (w < wave) base += t;
     all += t;
@@ -92,13 +92,18 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(const T* in, int64_
 
 // The whole scan in one workgroup, a tile at a time with a running carry:
 // one launch instead of three for the short scans (radix-sort digit counts
-// of small inputs, per-variable flags) whose cost is launch latency.
-constexpr int64_t kScanSingleMax = 16 * kScanTile;
+// of small inputs, per-variable flags) whose cost is launch latency. Each
+// tile is a dependent round trip, so longer scans take the three-launch
+// path (a 1,024-thread form of this kernel over 50k counts took 47 us
+// against about 10 us of kernel time for the three launches).
+constexpr int kScanSingleThreads = kScanThreads;
+constexpr int64_t kScanSingleTile = kScanTile;
+constexpr int64_t kScanSingleMax = 16 * kScanSingleTile;
 
 template <typename T>
-__global__ void __launch_bounds__(kScanThreads) k_scan_single(const T* in, int64_t n, T* out, T* total) {
+__global__ void __launch_bounds__(kScanSingleThreads) k_scan_single(const T* in, int64_t n, T* out, T* total) {
   T carry = 0;
-  for (int64_t t0 = 0; t0 < n; t0 += kScanTile) {
+  for (int64_t t0 = 0; t0 < n; t0 += kScanSingleTile) {
     const int64_t base = t0 + (int64_t)threadIdx.x * kScanItems;
     T v[kScanItems];
     T sum = 0;
@@ -109,7 +114,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_single(const T* in, int64
       sum += v[k];
     }
     T tot;
-    T run = block_exclusive_scan<T>(sum, &tot) + carry;
+    T run = block_exclusive_scan<T, kScanSingleThreads>(sum, &tot) + carry;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
       const int64_t i = base + k;
@@ -134,7 +139,7 @@ int exclusive_scan_impl(const T* in, T* out, int64_t n, T* total, hipStream_t s)
   }
   const int64_t nb = ceil_div(n, kScanTile);
   if (n <= kScanSingleMax) {
-    hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(kScanThreads), 0, s, in, n, out, total);
+    hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(kScanSingleThreads), 0, s, in, n, out, total);
     FH_HIP(hipGetLastError());
     return FH_OK;
   }
