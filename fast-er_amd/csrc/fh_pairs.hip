@@ -420,6 +420,13 @@ struct MergeArgs {
   // below it sets *err (the merge then fails with FH_ERR_ARG).
   uint32_t nlev[kMaxVars];
   uint32_t* err;
+  // Listed exact variables whose list is exactly their equal pairs (level 1)
+  // of the per-row value ids lid_a[w][row1], lid_b[w][row2] (negative: no
+  // value): other lists' entries take their level from the two ids instead
+  // of a search of the list (null: searched).
+  const int64_t* lid_a[kMaxVars];
+  const int64_t* lid_b[kMaxVars];
+  int n_lid;
 };
 
 __device__ __forceinline__ int64_t lower_bound_u64(const uint64_t* a, int64_t lo, int64_t hi, uint64_t x) {
@@ -463,8 +470,8 @@ __global__ void __launch_bounds__(kThreads) k_tile_bounds(MergeArgs m, int v, in
   if (g >= per * m.n_vars) return;
   const int w = (int)(g / per);
   const int64_t t = g - (int64_t)w * per;
-  int64_t b = 0;
-  if (w != v) {
+  int64_t b = 0;  // empty ranges for the source list and lists looked up by row ids
+  if (w != v && m.lid_a[w] == nullptr) {
     b = t == ntiles ? m.sizes[w]
                     : lower_bound_u64(m.lists[w], 0, m.sizes[w], (m.lists[v][t * kMergeTile] >> 2) * 4ull);
   }
@@ -672,20 +679,32 @@ __global__ void __launch_bounds__(kThreads) k_pattern_codes(MergeArgs m, int v, 
   // one LDS atomic per distinct code (lanes of one code found by eight
   // ballots), as a list's owners often share a few codes.
   const uint64_t lt = fhd::lanemask_lt();
+  bool lid_other = false;  // a list other than the source is looked up by row ids
+  for (int w = 0; w < nv; ++w) lid_other = lid_other || (w != v && m.lid_a[w] != nullptr);
 #pragma unroll
   for (int k = 0; k < kMergeBatch; ++k) {
     const int i = k * kThreads + tid;
     uint32_t code = 0xFFFFFFFFu;
     if (i < tn) {
       code = tcode[i];
-      if (tgone[i]) {
-        code = 0xFFFFFFFFu;
-      } else if (m.n_imp > 0) {
+      bool gone = tgone[i] != 0;
+      if (!gone && (m.n_imp > 0 || lid_other)) {
         const uint64_t id = tids[i];
         const uint64_t r1 = id / m.id_div, r2 = id % m.id_div;
+        for (int w = 0; w < m.n_vars; ++w) {
+          if (w == v || m.lid_a[w] == nullptr) continue;
+          const int64_t x = m.lid_a[w][r1];
+          if (x >= 0 && x == m.lid_b[w][r2]) {
+            if (w < v)
+              gone = true;  // the earlier list w holds the pair
+            else
+              code += m.stride[w];  // level 1 of w
+          }
+        }
         for (int q = 0; q < m.n_imp; ++q)
           if (m.imp_a[q][r1] == m.imp_b[q][r2]) code += m.imp_stride[q];
       }
+      if (gone) code = 0xFFFFFFFFu;
       codes[t0 + i] = code;
     }
     if (tile_hist) {
@@ -1308,13 +1327,26 @@ extern "C" int fh_sorted_member_mask(const int64_t* arr1, int64_t n1, const int6
 static int pattern_merge_impl(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
                               const int32_t* var_levels, int n_imp, const int64_t* const* imp_a,
                               const int64_t* const* imp_b, int64_t id_div, int64_t* ids, int64_t capacity,
-                              int64_t* pattern_offsets, int64_t* n_union, hipStream_t s) {
+                              int64_t* pattern_offsets, int64_t* n_union, hipStream_t s,
+                              const int64_t* const* lid_a = nullptr, const int64_t* const* lid_b = nullptr) {
   FH_REQUIRE(n_vars >= 1 && n_vars <= kMaxVars, "fh_pattern_merge: 1..%d listed variables supported", kMaxVars);
   FH_REQUIRE(n_imp >= 0 && n_imp <= kMaxVars, "fh_pattern_merge: 0..%d row-id variables supported", kMaxVars);
   FH_REQUIRE(pattern_offsets != nullptr && n_union != nullptr, "fh_pattern_merge: null host outputs");
   MergeArgs m;
   m.n_vars = n_vars;
   m.n_imp = n_imp;
+  m.n_lid = 0;
+  for (int v = 0; v < kMaxVars; ++v) {
+    m.lid_a[v] = nullptr;
+    m.lid_b[v] = nullptr;
+  }
+  for (int v = 0; v < n_vars && lid_a && lid_b; ++v) {
+    if (lid_a[v] == nullptr || lid_b[v] == nullptr) continue;
+    FH_REQUIRE(var_levels[v] == 2, "fh_pattern_merge_listed: variable %d with row ids must have 2 levels", v);
+    m.lid_a[v] = lid_a[v];
+    m.lid_b[v] = lid_b[v];
+    ++m.n_lid;
+  }
   m.id_div = (uint64_t)(id_div > 0 ? id_div : 1);
   uint64_t n_patterns = 1;
   for (int v = 0; v < n_vars; ++v) {
@@ -1499,6 +1531,17 @@ extern "C" int fh_pattern_merge_rowids(int n_vars, const uint64_t* const* var_ke
   FH_REQUIRE(id_div > 0, "fh_pattern_merge_rowids: id_div must be positive");
   return pattern_merge_impl(n_vars, var_keys, var_sizes, var_levels, n_exact, ids_a, ids_b, id_div, ids, capacity,
                             pattern_offsets, n_union, (hipStream_t)stream);
+}
+
+extern "C" int fh_pattern_merge_listed(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
+                                       const int32_t* var_levels, const int64_t* const* ids_a,
+                                       const int64_t* const* ids_b, int64_t id_div, int64_t* ids, int64_t capacity,
+                                       int64_t* pattern_offsets, int64_t* n_union, fh_stream_t stream) {
+  clear_error();
+  FH_REQUIRE(id_div > 0, "fh_pattern_merge_listed: id_div must be positive");
+  FH_REQUIRE(ids_a != nullptr && ids_b != nullptr, "fh_pattern_merge_listed: null row-id arrays");
+  return pattern_merge_impl(n_vars, var_keys, var_sizes, var_levels, 0, nullptr, nullptr, id_div, ids, capacity,
+                            pattern_offsets, n_union, (hipStream_t)stream, ids_a, ids_b);
 }
 
 extern "C" int fh_pair_posteriors(const int64_t* pattern_offsets, int n_patterns, const double* ksi, double* post,

@@ -91,6 +91,9 @@ _SIGS = {
                                        ctypes.POINTER(ctypes.c_int32), _i32, ctypes.POINTER(_vp),
                                        ctypes.POINTER(_vp), _i64, _vp, _i64, ctypes.POINTER(_i64),
                                        ctypes.POINTER(_i64), _vp]),
+    "fh_pattern_merge_listed": (_i32, [_i32, ctypes.POINTER(_vp), ctypes.POINTER(_i64),
+                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                       _i64, _vp, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64), _vp]),
     "fh_em_fit": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _f64, _i32, _vp,
                          ctypes.POINTER(_i32), ctypes.POINTER(_i32), _vp]),
     "fh_em_fit_noalloc": (_i32, [_i32, ctypes.POINTER(ctypes.c_int32), _vp, _vp, _f64, _i32, _vp,

@@ -359,12 +359,16 @@ def member_mask(arr1: torch.Tensor, arr2: torch.Tensor, inside: bool) -> torch.T
     return mask[:n1].bool()
 
 
-def pattern_merge(var_keys, var_levels, rowid_vars=None, id_div=1):
+def pattern_merge(var_keys, var_levels, rowid_vars=None, id_div=1, listed_ids=None):
     """Per-variable sorted level keys -> (ids, host pattern offsets, n_union).
 
     ``rowid_vars``: optional list of (ids_a, ids_b) device int64 tensors of
     two-level exact variables that follow the listed ones in pattern order and
-    are looked up per pair instead of being listed (fh_pattern_merge_rowids)."""
+    are looked up per pair instead of being listed (fh_pattern_merge_rowids).
+    ``listed_ids``: optional per listed variable (ids_a, ids_b) or None; a
+    pair of per-row value ids states that the variable's list is exactly the
+    pairs of equal ids, which the other lists then look up by id
+    (fh_pattern_merge_listed)."""
     L = _lib.load()
     nv = len(var_keys)
     keys = [k.contiguous() for k in var_keys]
@@ -382,6 +386,14 @@ def pattern_merge(var_keys, var_levels, rowid_vars=None, id_div=1):
         pb = (ctypes.c_void_p * ne)(*[b.data_ptr() for _, b in rowid_vars])
         check(L.fh_pattern_merge_rowids(nv, ptrs, sizes, levels, ne, pa, pb, _i64(id_div), ptr(ids), _i64(cap),
                                         offs, ctypes.byref(n_union), stream()), "fh_pattern_merge_rowids")
+    elif listed_ids is not None and any(x is not None for x in listed_ids):
+        assert len(listed_ids) == nv
+        la = [None if x is None else x[0].contiguous() for x in listed_ids]
+        lb = [None if x is None else x[1].contiguous() for x in listed_ids]
+        pa = (ctypes.c_void_p * nv)(*[0 if x is None else x.data_ptr() for x in la])
+        pb = (ctypes.c_void_p * nv)(*[0 if x is None else x.data_ptr() for x in lb])
+        check(L.fh_pattern_merge_listed(nv, ptrs, sizes, levels, pa, pb, _i64(id_div), ptr(ids), _i64(cap), offs,
+                                        ctypes.byref(n_union), stream()), "fh_pattern_merge_listed")
     else:
         check(L.fh_pattern_merge(nv, ptrs, sizes, levels, ptr(ids), _i64(cap), offs, ctypes.byref(n_union),
                                  stream()), "fh_pattern_merge")

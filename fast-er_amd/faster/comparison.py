@@ -358,11 +358,15 @@ class Comparison():
             self._var_keys = keys
             self._Fit_flag = True
             return self
-        self._finish(keys, n_a * n_b)
+        # The exact lists are the equal pairs of the value ids: the merge looks
+        # them up by id for the other lists' entries.
+        listed_ids = [None] * len(inputs.fuzzy) + [(g.group_a, g.group_b) for g in exact] + [None] * len(
+            inputs.numeric)
+        self._finish(keys, n_a * n_b, listed_ids, n_b)
         return self
 
-    def _finish(self, keys, n_pairs):
-        ids, offs, n_union = _ops.pattern_merge(keys, self._var_levels)
+    def _finish(self, keys, n_pairs, listed_ids=None, id_div=1):
+        ids, offs, n_union = _ops.pattern_merge(keys, self._var_levels, listed_ids=listed_ids, id_div=id_div)
         self._pattern_ids = ids
         self._pattern_offsets = offs
         self.Indices = [ids[offs[k]:offs[k + 1]] for k in range(1, len(offs) - 1)]

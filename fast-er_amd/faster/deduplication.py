@@ -214,7 +214,11 @@ class Deduplication():
             d2, d1 = float(Numeric_Dist[0]), float(Numeric_Dist[1])
             for x in inputs.numeric:
                 keys.append(_ops.numeric_dedup(x, d2, d1))
-            ids, offs, _ = _ops.pattern_merge(keys, self.Var_Levels)
+            # Without missing levels, the exact lists are the equal pairs of the
+            # value ids: the merge looks them up by id for the other lists.
+            listed_ids = None if inputs.missing else (
+                [None] * len(inputs.fuzzy) + [(g.group, g.group) for g in exact] + [None] * len(inputs.numeric))
+            ids, offs, _ = _ops.pattern_merge(keys, self.Var_Levels, listed_ids=listed_ids, id_div=n)
             if inputs.missing:
                 ids, offs, self._Counts, self.Indices = missing.apply_missing(
                     ids, offs, base_levels, inputs.mask, inputs.mask, inputs.mask_dev, inputs.mask_dev, n,

@@ -433,6 +433,25 @@ int fh_pattern_merge_rowids(int n_vars, const uint64_t* const* var_keys, const i
                             int64_t* pattern_offsets, int64_t* n_union, fh_stream_t stream);
 
 /*
+ * fh_pattern_merge with a lookup aid for listed exact variables.
+ *
+ * The same merge and outputs as fh_pattern_merge (faster/comparison.py:
+ * 695-725, faster/deduplication.py:773-803) over the same n_vars lists. For
+ * a variable v with ids_a[v] and ids_b[v] non-null (device int64 per-row
+ * value ids; a negative id is a row without a value), the caller states
+ * that v has 2 levels and that its list is exactly the pairs with
+ * ids_a[v][id / id_div] == ids_b[v][id % id_div] >= 0, at level 1 (the
+ * exact-equality lists of faster/comparison.py:500-600 and
+ * faster/deduplication.py:628-701; for deduplication ids_b[v] = ids_a[v]
+ * and id_div = n). The other lists' entries then take v's level from the two
+ * ids instead of a search of v's list; null entries are searched as usual.
+ */
+int fh_pattern_merge_listed(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
+                            const int32_t* var_levels, const int64_t* const* ids_a, const int64_t* const* ids_b,
+                            int64_t id_div, int64_t* ids, int64_t capacity, int64_t* pattern_offsets,
+                            int64_t* n_union, fh_stream_t stream);
+
+/*
  * Fellegi-Sunter EM over a pattern histogram, FP64, one workgroup running
  * every iteration on the device.
  *

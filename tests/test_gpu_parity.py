@@ -458,6 +458,80 @@ def test_pattern_merge_lists_of_very_different_sizes(levels, sizes):
     expect = np.concatenate([[0], np.cumsum(np.bincount(code, minlength=n_pat))])
     np.testing.assert_array_equal(np.asarray(list(offs)), expect)
 
+
+def test_pattern_merge_listed_exact_ids_equal_plain_merge():
+    """fh_pattern_merge_listed through the C ABI: exact variables whose lists
+    are the equal pairs of per-row value ids (negative ids: rows without a
+    value), given with those ids, merge to the same ids and offsets as
+    fh_pattern_merge over the same lists, with the exact variables first,
+    between and last in pattern order; a row-id variable with 3 levels is
+    rejected."""
+    import ctypes
+
+    from faster import _lib
+
+    L = _lib.load()
+    rng = np.random.default_rng(17)
+    dev = torch.device("cuda")
+    n_a, n_b = 2000, 3000
+    ga = rng.integers(-1, 40, n_a)
+    gb = rng.integers(-1, 40, n_b)
+    ii, jj = np.nonzero((ga[:, None] == gb[None, :]) & (ga[:, None] >= 0))
+    exact_keys = (ii.astype(np.int64) * n_b + jj) * 4 + 1
+    ga2 = rng.integers(0, 7, n_a)
+    gb2 = rng.integers(0, 7, n_b)
+    ii2, jj2 = np.nonzero(ga2[:, None] == gb2[None, :])
+    exact2_keys = (ii2.astype(np.int64) * n_b + jj2) * 4 + 1
+
+    def fuzzy(m):
+        ids = np.unique(rng.integers(0, n_a * n_b, m))
+        return ids * 4 + rng.integers(1, 3, len(ids))
+
+    tens = lambda a: torch.as_tensor(np.asarray(a, dtype=np.int64), device=dev)
+    ids_t = {"e": (tens(ga), tens(gb)), "e2": (tens(ga2), tens(gb2))}
+    lists = {"e": exact_keys, "e2": exact2_keys, "f1": fuzzy(400_000), "f2": fuzzy(30_000)}
+    for order in (["e", "f1", "f2"], ["f1", "e", "f2", "e2"], ["f1", "f2", "e"]):
+        keys = [tens(lists[x]) for x in order]
+        levels = [2 if x.startswith("e") else 3 for x in order]
+        nv = len(keys)
+        total = sum(k.numel() for k in keys)
+        n_pat = int(np.prod(levels))
+        ptrs = (ctypes.c_void_p * nv)(*[k.data_ptr() for k in keys])
+        sizes = (ctypes.c_int64 * nv)(*[k.numel() for k in keys])
+        lv = (ctypes.c_int32 * nv)(*levels)
+        outs = []
+        for listed in (False, True):
+            out = torch.full((total,), -7, dtype=torch.int64, device=dev)
+            offs = (ctypes.c_int64 * (n_pat + 1))()
+            nu = ctypes.c_int64(-1)
+            if listed:
+                pa = (ctypes.c_void_p * nv)(*[ids_t[x][0].data_ptr() if x in ids_t else 0 for x in order])
+                pb = (ctypes.c_void_p * nv)(*[ids_t[x][1].data_ptr() if x in ids_t else 0 for x in order])
+                st = L.fh_pattern_merge_listed(nv, ptrs, sizes, lv, pa, pb, ctypes.c_int64(n_b), _lib.ptr(out),
+                                               ctypes.c_int64(total), offs, ctypes.byref(nu), _lib.stream())
+            else:
+                st = L.fh_pattern_merge(nv, ptrs, sizes, lv, _lib.ptr(out), ctypes.c_int64(total), offs,
+                                        ctypes.byref(nu), _lib.stream())
+            torch.cuda.synchronize()
+            assert st == 0, L.fh_last_error()
+            outs.append((nu.value, out.cpu().numpy()[:nu.value], list(offs)))
+        assert outs[0][0] == outs[1][0]
+        np.testing.assert_array_equal(outs[0][1], outs[1][1])
+        assert outs[0][2] == outs[1][2]
+        union = np.unique(np.concatenate([lists[x] >> 2 for x in order]))
+        assert outs[1][0] == len(union)
+    bad = (ctypes.c_int32 * 2)(3, 3)
+    k2 = [tens(lists["e"]), tens(lists["f1"])]
+    ptrs = (ctypes.c_void_p * 2)(*[k.data_ptr() for k in k2])
+    sizes = (ctypes.c_int64 * 2)(*[k.numel() for k in k2])
+    pa = (ctypes.c_void_p * 2)(ids_t["e"][0].data_ptr(), 0)
+    pb = (ctypes.c_void_p * 2)(ids_t["e"][1].data_ptr(), 0)
+    out = torch.empty(sum(k.numel() for k in k2), dtype=torch.int64, device=dev)
+    offs = (ctypes.c_int64 * 10)()
+    nu = ctypes.c_int64(-1)
+    assert L.fh_pattern_merge_listed(2, ptrs, sizes, bad, pa, pb, ctypes.c_int64(n_b), _lib.ptr(out),
+                                     ctypes.c_int64(out.numel()), offs, ctypes.byref(nu), _lib.stream()) != 0
+
 def test_deduplication_fit(oracle):
     import faster
     from faster.synthetic import dedup_table
