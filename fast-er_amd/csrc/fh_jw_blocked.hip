@@ -116,8 +116,8 @@ __global__ void __launch_bounds__(256) k_blocked_hist(const unsigned long long* 
     atomicAdd(hist + (uint64_t)block_a[i] * n_patterns + code, 1ull);
     ++mine;
   }
-  for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o, 64);
-  if ((threadIdx.x & 63) == 0 && mine) atomicAdd(n_union, mine);
+  mine = fhd::block_reduce(mine, fhd::OpSum());
+  if (threadIdx.x == 0 && mine) atomicAdd(n_union, mine);
 }
 
 // Sort keys for one side: block * kNumClasses + length class; presence bits.
@@ -259,7 +259,7 @@ struct BlockedPlan {
   }
 
   int enqueue_hist() {
-    const int64_t hb = std::min<int64_t>(ceil_div((int64_t)hslots, 256), 16384);
+    const int64_t hb = std::min<int64_t>(ceil_div((int64_t)hslots, 256), 2048);
     hipLaunchKernelGGL(k_blocked_hist, dim3((unsigned)hb), dim3(256), 0, stream, hkeys, hcodes, hslots, (uint64_t)n_b,
                        block_a, ex, n_patterns, hist, n_union);
     FH_HIP(hipGetLastError());
