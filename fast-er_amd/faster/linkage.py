@@ -38,10 +38,6 @@ class Linkage():
         (faster/linkage.py:54), i.e. the last pattern's list is linked when
         pattern 0's probability reaches the threshold; ``Compat=False`` skips
         pattern 0 instead (its pairs are never listed)."""
-        df_A = self.df_A.add_suffix("_A")
-        df_B = self.df_B.add_suffix("_B")
-        df_A["Index_A"] = range(len(df_A))
-        df_B["Index_B"] = range(len(df_B))
         above = np.ravel(np.argwhere(np.asarray(self.Ksi) >= Threshold))
         if not Compat:
             above = above[above > 0]
@@ -55,9 +51,23 @@ class Linkage():
                 x = x.materialize()
             parts.append(x if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x, dtype=np.int64)))
         ids = torch.cat([x.to(dev, torch.int64) for x in parts])
-        n_b = len(df_B)
+        n_b = len(self.df_B)
         a = (ids // n_b).cpu().numpy()
         b = (ids % n_b).cpu().numpy()
-        df_A = df_A.iloc[a, :]
-        df_A["Index_B"] = b
-        return df_A.merge(df_B, on="Index_B")
+        return pairs_frame(self.df_A, self.df_B, a, b)
+
+
+def pairs_frame(df_A: pd.DataFrame, df_B: pd.DataFrame, a, b) -> pd.DataFrame:
+    """The reference's result for the pairs (a[k], b[k]) (faster/linkage.py:
+    44-72): the rows of df_A and df_B with the suffixes _A and _B, Index_A,
+    Index_B, in pair order, on a fresh RangeIndex. The reference joins on
+    Index_B, which is unique on the right (it overwrites any column of that
+    name) and keeps the left order; row-aligned takes of both frames give the
+    same frame without the join."""
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    left = df_A.iloc[a].add_suffix("_A").reset_index(drop=True)
+    left["Index_A"] = a
+    left["Index_B"] = b
+    right = df_B.iloc[b].add_suffix("_B").reset_index(drop=True).drop(columns=["Index_B"], errors="ignore")
+    return pd.concat([left, right], axis=1)

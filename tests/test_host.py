@@ -216,3 +216,36 @@ def test_output_size_memory_ignores_much_smaller_calls():
     for k in range(40):
         r.add(10 ** 6, 1e-3)
     assert len(r.seen) == 32 and r.predict(10 ** 6) == 1e-3
+
+
+def test_linkage_pairs_frame_equals_the_reference_join():
+    """Linkage.transform's frame (pairs_frame, row-aligned takes) equals the
+    reference's construction (faster/linkage.py:44-72: suffixes, Index_A and
+    Index_B columns, iloc, merge on Index_B), with non-default index labels, a
+    column of df_B that becomes Index_B, repeated rows and an empty selection."""
+    import warnings
+
+    import pandas as pd
+
+    from faster.linkage import pairs_frame
+
+    rng = np.random.default_rng(8)
+    n_a, n_b = 300, 200
+    df_A = pd.DataFrame({"name": rng.integers(0, 50, n_a).astype(str), "year": rng.integers(1930, 2000, n_a),
+                         "w": rng.random(n_a)}, index=rng.permutation(n_a) + 1000)
+    df_B = pd.DataFrame({"name": rng.integers(0, 50, n_b).astype(str), "Index": rng.integers(0, 9, n_b),
+                         "w": rng.random(n_b)}, index=rng.permutation(n_b) * 3)
+    for m in (0, 1, 500):
+        a = rng.integers(0, n_a, m)
+        b = rng.integers(0, n_b, m)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            ra = df_A.add_suffix("_A")
+            rb = df_B.add_suffix("_B")
+            ra["Index_A"] = range(len(ra))
+            rb["Index_B"] = range(len(rb))
+            ra = ra.iloc[a, :]
+            ra["Index_B"] = b
+            ref = ra.merge(rb, on="Index_B")
+        got = pairs_frame(df_A, df_B, a, b)
+        pd.testing.assert_frame_equal(got, ref, check_index_type=False)
