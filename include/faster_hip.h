@@ -445,6 +445,9 @@ int fh_pattern_merge_rowids(int n_vars, const uint64_t* const* var_keys, const i
  * faster/deduplication.py:628-701; for deduplication ids_b[v] = ids_a[v]
  * and id_div = n). The other lists' entries then take v's level from the two
  * ids instead of a search of v's list; null entries are searched as usual.
+ * Every pair id of every list must index both arrays (id / id_div below the
+ * length of ids_a[v], id % id_div below that of ids_b[v]), as the lists of
+ * those rows' pairs do.
  */
 int fh_pattern_merge_listed(int n_vars, const uint64_t* const* var_keys, const int64_t* var_sizes,
                             const int32_t* var_levels, const int64_t* const* ids_a, const int64_t* const* ids_b,
