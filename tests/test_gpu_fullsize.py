@@ -41,10 +41,10 @@ def test_config1_10k_all_patterns_and_em(oracle):
 
     n = 10_000
     dfa, dfb, _ = linkage_tables(n, n, seed=0)
-    print("config 1: 10k x 10k Comparison.fit", flush=True)
+    print("10k x 10k Comparison.fit", flush=True)
     comp = faster.Comparison(dfa, dfb, VF, VF, ["house_number"], ["house_number"])
     comp.fit()
-    print("config 1: oracle", flush=True)
+    print("10k x 10k oracle", flush=True)
     _, indices, counts = oracle.comparison_fit([dfa[v].to_numpy() for v in VF], [dfb[v].to_numpy() for v in VF],
                                                [dfa["house_number"].to_numpy()], [dfb["house_number"].to_numpy()])
     assert len(comp.Indices) == len(indices) == 53
@@ -71,7 +71,7 @@ def fullsize():
     from faster.synthetic import linkage_tables
 
     dfa, dfb, _ = linkage_tables(N, N, seed=0)
-    print("config 2: 1M x 1M Comparison.fit", flush=True)
+    print("1M x 1M Comparison.fit", flush=True)
     comp = faster.Comparison(dfa, dfb, VF, VF, ["house_number"], ["house_number"])
     comp.fit()
     assert comp.Exact_Mode == "list"
@@ -181,7 +181,7 @@ def test_config3_dedup_2m(oracle):
 
     n = 2_000_000
     df, _ = dedup_table(n, seed=0)
-    print("config 3: 2M deduplication", flush=True)
+    print("2M-row deduplication", flush=True)
     dd = faster.Deduplication(df, VF, ["birth_year"])
     dd.fit()
     assert dd.Exact_Mode == "rowid"

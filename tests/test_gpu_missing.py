@@ -158,7 +158,7 @@ def test_config5_histogram_invariants(config5):
         # Pairs at each listed level of v, over all patterns, are v's level list.
         for l in range(1, comp.Var_Levels[v] - 1):
             assert counts[G[:, v] == l].sum() == comp.Levels[v][l - 1].numel(), (name, l)
-    print("config 5 histogram: " + " ".join(str(int(c)) for c in counts[:8]) + " ...", flush=True)
+    print("768-pattern histogram of the 1M x 1M records: " + " ".join(str(int(c)) for c in counts[:8]) + " ...", flush=True)
 
 
 def test_config5_sampled_rows_vs_oracle(oracle, config5):
