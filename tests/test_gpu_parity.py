@@ -109,6 +109,41 @@ def test_all_pairs_levels(oracle, seed, alpha, lo_len, hi_len):
     _eq_lists(got, ref, f"seed {seed}")
 
 
+@pytest.mark.parametrize("mode", ["link", "dedup"])
+@pytest.mark.parametrize("lens,alpha,n", [((1, 2, 3), "abcdef", 600), ((4, 5), "abcdefgh", 3000),
+                                          ((5, 6, 7, 8), "abcd", 2600),
+                                          ((10, 11, 15, 16), "abc", 1800), ((31, 32, 33), "ab", 900),
+                                          ((63, 64, 65), "ab", 500)])
+def test_mask_kind_boundaries(oracle, mode, lens, alpha, n):
+    """Values at the length boundaries of the mask kinds (5 strings per dword
+    up to 5 bytes, 4 up to 7, 3 up to 10, 2 up to 15, one 32-bit mask up to
+    32, one 64-bit mask up to 64, the byte-wise path beyond), over alphabets
+    small enough that every class holds near copies and long shared
+    prefixes, with classes of more than one string-1 wave and more than one
+    string-2 tile (1,280 five-byte strings fill a tile of the five-string
+    kind)."""
+    from faster.comparison import jaro_winkler_unique_gpu
+    from faster.deduplication import jaro_winkler_dedup_gpu
+
+    rng = np.random.default_rng(sum(lens) * 7 + len(alpha))
+
+    def draw(m):
+        return ["".join(alpha[k] for k in rng.integers(0, len(alpha), int(rng.choice(lens)))) for _ in range(m)]
+
+    def near(s):
+        b = list(s)
+        b[int(rng.integers(0, len(b)))] = alpha[int(rng.integers(0, len(alpha)))]
+        return "".join(b)
+
+    A = draw(n)
+    B = draw(n) + [near(s) for s in A[: n // 3]]
+    if mode == "link":
+        _eq_lists(jaro_winkler_unique_gpu(A, B), oracle.jaro_winkler_unique(A, B), f"link {lens}")
+    else:
+        S = sorted(set(A + B[n:]))
+        _eq_lists(jaro_winkler_dedup_gpu(S), oracle.jaro_winkler_dedup(S), f"dedup {lens}")
+
+
 @pytest.mark.parametrize("lo,hi", [(0.88, 0.94), (0.0, 0.5)])
 def test_values_of_any_length(oracle, lo, hi):
     """Values over 1,024 bytes (scored with their flags in global scratch)
