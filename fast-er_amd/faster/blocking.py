@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from ._lib import check, stream
-from .encode import group_exact_link, pack_strings
+from .encode import exact_column, group_exact_link_device, pack_column
 
 DEFAULT_TUNING = dict(tile_width=256, tiles_per_item=4, waves_per_item=8, wgs_per_cu=2, hash_slots=0)
 
@@ -33,11 +33,57 @@ _TUNE_CACHE = {}
 
 
 def _packed_device(values):
-    chars, offs = pack_strings(values)
+    chars, offs = pack_column(values)
     dev = _lib.device()
     if chars.size == 0:
         chars = np.zeros(1, np.uint8)
     return torch.from_numpy(chars).to(dev), torch.from_numpy(offs).to(dev)
+
+
+def _equal_pairs_per_block(block_a, block_b, cols_a, cols_b, n_blocks):
+    """Per block b: the record pairs (i in A, j in B) of block b with
+    cols_a[q][i] == cols_b[q][j] for every q, as exact int64 sums of
+    (A rows) x (B rows) over the groups of equal (block, values)."""
+    key_a = np.asarray(block_a, dtype=np.int64)
+    key_b = np.asarray(block_b, dtype=np.int64)
+    span = 1  # product of the value ranges: key = block * span + mixed-radix values
+    for ca, cb in zip(cols_a, cols_b):
+        m = int(max(ca.max(initial=-1), cb.max(initial=-1))) + 1
+        if m <= 0:
+            return np.zeros(n_blocks, dtype=np.int64)
+        if span > ((1 << 62) // max(n_blocks, 1)) // m:
+            return _equal_pairs_per_block_rows(block_a, block_b, cols_a, cols_b, n_blocks)
+        key_a = key_a * m + ca
+        key_b = key_b * m + cb
+        span *= m
+    ua, na = np.unique(key_a, return_counts=True)
+    ub, nb = np.unique(key_b, return_counts=True)
+    out = np.zeros(n_blocks, dtype=np.int64)
+    if ua.size == 0 or ub.size == 0:
+        return out
+    pos = np.minimum(np.searchsorted(ub, ua), ub.size - 1)
+    hit = ub[pos] == ua
+    np.add.at(out, ua[hit] // span, na[hit].astype(np.int64) * nb[pos[hit]].astype(np.int64))
+    return out
+
+
+def _equal_pairs_per_block_rows(block_a, block_b, cols_a, cols_b, n_blocks):
+    """_equal_pairs_per_block for value ranges whose mixed-radix key would
+    not fit in int64: groups from np.unique over the rows of (block, values)."""
+    both = np.concatenate([np.stack([np.asarray(block_a, dtype=np.int64)] + list(cols_a), 1),
+                           np.stack([np.asarray(block_b, dtype=np.int64)] + list(cols_b), 1)])
+    _, inv = np.unique(both, axis=0, return_inverse=True)
+    inv = np.asarray(inv).reshape(-1)
+    g = int(inv.max()) + 1 if inv.size else 0
+    n_a = len(block_a)
+    ca = np.bincount(inv[:n_a], minlength=g).astype(np.int64)
+    cb = np.bincount(inv[n_a:], minlength=g).astype(np.int64)
+    blk = np.zeros(g, dtype=np.int64)
+    blk[inv[:n_a]] = block_a
+    blk[inv[n_a:]] = block_b
+    out = np.zeros(n_blocks, dtype=np.int64)
+    np.add.at(out, blk, ca * cb)
+    return out
 
 
 def exact_totals_per_block(ids_a, ids_b, block_a, block_b, n_blocks):
@@ -50,18 +96,8 @@ def exact_totals_per_block(ids_a, ids_b, block_a, block_b, n_blocks):
     N = {(): na_b * nb_b}
     for r in range(1, k + 1):
         for S in itertools.combinations(range(k), r):
-            cols_a = [block_a.astype(np.int64)] + [ids_a[q] for q in S]
-            cols_b = [block_b.astype(np.int64)] + [ids_b[q] for q in S]
-            both = np.concatenate([np.stack(cols_a, 1), np.stack(cols_b, 1)])
-            _, inv = np.unique(both, axis=0, return_inverse=True)
-            inv = np.asarray(inv).reshape(-1)
-            g = inv.max() + 1
-            ca = np.bincount(inv[:len(block_a)], minlength=g).astype(np.int64)
-            cb = np.bincount(inv[len(block_a):], minlength=g).astype(np.int64)
-            blk = np.zeros(g, dtype=np.int64)
-            blk[inv[:len(block_a)]] = block_a
-            blk[inv[len(block_a):]] = block_b
-            N[S] = np.bincount(blk, weights=(ca * cb).astype(np.float64), minlength=n_blocks).astype(np.int64)
+            N[S] = _equal_pairs_per_block(block_a, block_b, [np.asarray(ids_a[q], dtype=np.int64) for q in S],
+                                          [np.asarray(ids_b[q], dtype=np.int64) for q in S], n_blocks)
     out = {}
     for d in itertools.product(range(2), repeat=k):
         S = tuple(q for q in range(k) if d[q])
@@ -99,9 +135,12 @@ class BlockedLinkage:
         dev = _lib.device()
         self._a = [_packed_device(df_A[v].to_numpy()) for v in Vars_Fuzzy_A]
         self._b = [_packed_device(df_B[v].to_numpy()) for v in Vars_Fuzzy_B]
-        ex = [group_exact_link(df_A[a].to_numpy(), df_B[b].to_numpy()) for a, b in zip(Vars_Exact_A, Vars_Exact_B)]
-        self._ex_host = [(g.group_a, g.group_b) for g in ex]
-        self._ex_dev = [(torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)) for a, b in self._ex_host]
+        # Joint value ids of each exact column, grouped on the device; the
+        # host copies feed the closed-form per-block counts.
+        ex = [group_exact_link_device(exact_column(df_A[a].to_numpy(), df_B[b].to_numpy()), self.n_a)
+              for a, b in zip(Vars_Exact_A, Vars_Exact_B)]
+        self._ex_dev = [(g.group_a.to(torch.int64).contiguous(), g.group_b.to(torch.int64).contiguous()) for g in ex]
+        self._ex_host = [(a.cpu().numpy(), b.cpu().numpy()) for a, b in self._ex_dev]
         self._blk_a = torch.from_numpy(self.block_a).to(dev)
         self._blk_b = torch.from_numpy(self.block_b).to(dev)
         self.n_fuzzy = len(Vars_Fuzzy_A)

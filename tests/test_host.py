@@ -249,3 +249,33 @@ def test_linkage_pairs_frame_equals_the_reference_join():
             ref = ra.merge(rb, on="Index_B")
         got = pairs_frame(df_A, df_B, a, b)
         pd.testing.assert_frame_equal(got, ref, check_index_type=False)
+
+
+@pytest.mark.parametrize("k,huge", [(1, False), (2, False), (3, False), (2, True)])
+def test_exact_totals_per_block_brute_force(k, huge):
+    """Per-block counts of record pairs by exact-variable agreement pattern
+    (BlockedLinkage's closed-form counts) equal a count over every record
+    pair of every block; `huge` value ids overflow the int64 mixed-radix key
+    and take the row-grouping path."""
+    from faster.blocking import exact_totals_per_block
+
+    rng = np.random.default_rng(5 + k)
+    n_a, n_b, n_blocks = 260, 230, 7
+    ba = rng.integers(0, n_blocks - 1, n_a).astype(np.int32)  # block n_blocks - 1 stays empty
+    bb = rng.integers(0, n_blocks - 1, n_b).astype(np.int32)
+    scale = (1 << 40) if huge else 1
+    ids_a = [rng.integers(0, 4, n_a).astype(np.int64) * scale + q for q in range(k)]
+    ids_b = [rng.integers(0, 4, n_b).astype(np.int64) * scale + q for q in range(k)]
+    got = exact_totals_per_block(ids_a, ids_b, ba, bb, n_blocks)
+    want = {d: np.zeros(n_blocks, dtype=np.int64) for d in range(2 ** k)}
+    for i in range(n_a):
+        for j in range(n_b):
+            if ba[i] != bb[j]:
+                continue
+            d = 0
+            for q in range(k):
+                d = 2 * d + int(ids_a[q][i] == ids_b[q][j])
+            want[d][ba[i]] += 1
+    assert sorted(got) == sorted(want)
+    for d in want:
+        np.testing.assert_array_equal(got[d], want[d], err_msg=f"pattern {d}")
