@@ -508,7 +508,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     stats->pairs_pruned = (int64_t)pruned;
     stats->alg_lane_ops = ops;
     stats->frozen_model_ops = frozen;
-    stats->method = method;
+    stats->method = prefix ? FH_JW_METHOD_PREFIX : FH_JW_METHOD_TILES;  // the form the plan runs
   }
 
   // Upload plan, tables.

@@ -93,7 +93,10 @@ typedef struct {
  *     string-1 values walked one after another by each wave, whose lanes
  *     cover the string-2 tile; the match state of a prefix shared with the
  *     previous (sorted) value is reused. String-2 values longer than 32
- *     bytes (and values over 64 bytes) use the TILES kernel.
+ *     bytes (and values over 64 bytes) use the TILES kernel, and so does the
+ *     whole call when more than about 55 byte values occur on both sides
+ *     (the prefix form's 256-dword tile then exceeds the LDS); the stats'
+ *     `method` reports the form that ran.
  * No counterpart in the reference (one CUDA kernel scores every pair,
  * faster/comparison.py:133-186); the switch exists to time and check the
  * forms against each other.

@@ -144,6 +144,46 @@ def test_mask_kind_boundaries(oracle, mode, lens, alpha, n):
         _eq_lists(jaro_winkler_dedup_gpu(S), oracle.jaro_winkler_dedup(S), f"dedup {lens}")
 
 
+@pytest.mark.parametrize("n_sym", [60, 120, 0])
+def test_large_alphabets(oracle, n_sym):
+    """Alphabets of more byte values than a 256-dword tile holds rows for, so
+    the per-lane form's tiles narrow to 128, 64 and (n_sym = 0: ASCII 1-127,
+    two-byte characters covering every lead and continuation byte, and one
+    three-byte character per lead byte, about 237 byte values on both sides)
+    32 dwords; linkage and deduplication, with near copies."""
+    from faster import _lib, _ops
+    from faster.comparison import jaro_winkler_unique_gpu
+    from faster.deduplication import jaro_winkler_dedup_gpu
+
+    rng = np.random.default_rng(n_sym)
+    if n_sym:
+        syms = [chr(c) for c in range(1, 128)] + [chr(c) for c in range(0xA0, 0x800)]
+        syms = [syms[int(k)] for k in rng.choice(len(syms), n_sym, replace=False)]
+    else:
+        syms = ([chr(c) for c in range(1, 128)] + [chr(((2 + k % 30) << 6) | ((7 * k) % 64)) for k in range(128)]
+                + [chr((j << 12) | (0x900 if j == 0 else 0x100)) for j in range(16)])
+
+    def draw(m):
+        return ["".join(syms[int(k)] for k in rng.integers(0, len(syms), int(rng.integers(1, 21)))) for _ in range(m)]
+
+    def near(s):
+        b = list(s)
+        b[int(rng.integers(0, len(b)))] = syms[int(rng.integers(0, len(syms)))]
+        return "".join(b)
+
+    A = draw(500)
+    B = draw(400) + [near(s) for s in A[:200]] + A[200:260]
+    _eq_lists(jaro_winkler_unique_gpu(A, B), oracle.jaro_winkler_unique(A, B), f"link {n_sym} symbols")
+    dev = _lib.device()
+    ca, oa = _pack(A, dev)
+    cb, ob = _pack(B, dev)
+    _, st = _ops.jw_levels(ca, oa, len(A), cb, ob, len(B), _lib.FH_MODE_LINK, 0.1, 0.88, 0.94, want_stats=True)
+    assert st["alphabet"] > (220 if n_sym == 0 else 55)  # tiles of 32 dwords / narrower than 256
+    assert st["method"] == _lib._METHODS["tiles"]
+    S = sorted(set(A + B[400:]))
+    _eq_lists(jaro_winkler_dedup_gpu(S), oracle.jaro_winkler_dedup(S), f"dedup {n_sym} symbols")
+
+
 @pytest.mark.parametrize("lo,hi", [(0.88, 0.94), (0.0, 0.5)])
 def test_values_of_any_length(oracle, lo, hi):
     """Values over 1,024 bytes (scored with their flags in global scratch)
