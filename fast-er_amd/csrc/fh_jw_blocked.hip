@@ -378,8 +378,18 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
   int max_krows = 1;
   pl->args.resize(n_fuzzy);
 
-  // Per-variable scratch for the sort (temporary) and per-variable persistent
-  // buffers (sorted rows, waves, tiles, tables).
+  // First, per variable: both sides sorted by (block, length class) into
+  // persistent row orders, their class histograms and the variable's
+  // alphabet. The tile width follows from the largest alphabet; then, per
+  // variable, the waves, tiles, work items and tables.
+  struct VarSort {
+    uint64_t* ent_a = nullptr;
+    uint64_t* ent_b = nullptr;
+    std::vector<uint32_t> ha, hb;
+    std::vector<uint8_t> amap;
+    int krows = 1;
+  };
+  std::vector<VarSort> sorts(n_fuzzy);
   for (int v = 0; v < n_fuzzy; ++v) {
     Scratch scr(s);
     uint32_t *key_a, *key_b, *key_tmp, *meta;
@@ -423,6 +433,28 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
       amap[b] = both ? (uint8_t)k++ : (uint8_t)K;
     }
     max_krows = std::max(max_krows, krows);
+    VarSort& vs = sorts[v];
+    vs.ent_a = ent_a;
+    vs.ent_b = ent_b;
+    vs.ha.swap(ha);
+    vs.hb.swap(hb);
+    vs.amap.swap(amap);
+    vs.krows = krows;
+  }
+  // Tile width: the widest power of two up to the requested one whose tile
+  // rows (every variable's alphabet) and survivor queues fit 64 KB of LDS.
+  const size_t qbytes = (size_t)kWaves * kQCap * sizeof(uint64_t);
+  auto lds_for = [&](int w) { return ((size_t)max_krows * (w + 1) + 1) * 4 + qbytes; };
+  while (pl->tw > 32 && lds_for(pl->tw) > 64 * 1024) pl->tw /= 2;
+  pl->lds_bytes = lds_for(pl->tw);
+  FH_REQUIRE(pl->lds_bytes <= 64 * 1024, "fh_blocked_plan_create: alphabet of %d symbols too large", max_krows - 1);
+  for (int v = 0; v < n_fuzzy; ++v) {
+    uint64_t* const ent_a = sorts[v].ent_a;
+    uint64_t* const ent_b = sorts[v].ent_b;
+    const std::vector<uint32_t>& ha = sorts[v].ha;
+    const std::vector<uint32_t>& hb = sorts[v].hb;
+    const std::vector<uint8_t>& amap = sorts[v].amap;
+    const int krows = sorts[v].krows;
     // Waves and tiles per (block, length class), in sorted order.
     std::vector<WaveMeta> waves;
     std::vector<TileMeta> tiles;
@@ -657,9 +689,6 @@ extern "C" int fh_blocked_plan_create(int n_fuzzy, const uint8_t* const* chars_a
     pl->long_grid = (int)wgs;
     FH_TRY(pl->alloc(&pl->long_scratch, (size_t)(wgs * 256 * words)));
   }
-  const size_t qbytes = (size_t)kWaves * kQCap * sizeof(uint64_t);
-  pl->lds_bytes = ((size_t)max_krows * (pl->tw + 1) + 1) * 4 + qbytes;
-  FH_REQUIRE(pl->lds_bytes <= 64 * 1024, "fh_blocked_plan_create: alphabet too large for tile width %d", pl->tw);
   int dev = 0, cus = 0;
   FH_HIP(hipGetDevice(&dev));
   FH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

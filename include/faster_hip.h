@@ -163,7 +163,9 @@ int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, int64_t n_a,
  * beyond). Empty strings pair at level 0; the thresholds must be above 0.
  */
 typedef struct {
-  int32_t tile_width;     /* string-2 tile width in dwords: 32, 64, 128 or 256 */
+  int32_t tile_width;     /* largest string-2 tile width in dwords: 32, 64, 128 or 256; the plan
+                             halves it while the tile rows of the largest alphabet and the
+                             survivor queues exceed 64 KB of LDS */
   int32_t tiles_per_item; /* tiles per work item */
   int32_t waves_per_item; /* string-1 waves per work item (1..8) */
   int32_t wgs_per_cu;     /* persistent workgroups per compute unit */

@@ -6,6 +6,7 @@ import json
 import os
 
 import numpy as np
+import pandas as pd
 import pytest
 import torch
 
@@ -779,6 +780,46 @@ def test_blocked_linkage_matches_per_block_loop(oracle):
     assert res
     bl.fit()
     np.testing.assert_array_equal(bl.Block_Counts, ref)
+
+
+def test_blocked_linkage_large_alphabet(oracle):
+    """Blocks whose values use about 80 byte values on both sides (mixed
+    case, digits, punctuation): the plan narrows its tiles below the
+    requested 256 dwords so the tile rows fit the LDS, and the batched launch,
+    the per-block launch loop, the per-block recipe and the oracle agree."""
+    from faster.blocking import BlockedLinkage, per_block_loop
+
+    rng = np.random.default_rng(17)
+    syms = [chr(c) for c in range(0x21, 0x7F)][:80]
+    n_blocks, n = 6, 2400
+
+    def word():
+        return "".join(syms[int(k)] for k in rng.integers(0, len(syms), int(rng.integers(2, 14))))
+
+    def near(w):
+        b = list(w)
+        b[int(rng.integers(0, len(b)))] = syms[int(rng.integers(0, len(syms)))]
+        return "".join(b)
+
+    cols_a = {v: [word() for _ in range(n)] for v in ("x", "y")}
+    cols_b = {v: [near(w) if k % 3 == 0 else word() for k, w in enumerate(cols_a[v])] for v in ("x", "y")}
+    dfa = pd.DataFrame({**cols_a, "e": rng.integers(0, 5, n)})
+    dfb = pd.DataFrame({**cols_b, "e": rng.integers(0, 5, n)})
+    ba = rng.integers(0, n_blocks, n).astype(np.int32)
+    bb = ba.copy()
+    bb[::7] = rng.integers(0, n_blocks, len(bb[::7]))
+    vf = ["x", "y"]
+    bl = BlockedLinkage(dfa, dfb, ba, bb, vf, vf, ["e"], ["e"])
+    ref, _ = per_block_loop(dfa, dfb, ba, bb, vf, ["e"])
+    bl.fit(use_graph=True)
+    np.testing.assert_array_equal(bl.Block_Counts, ref)
+    bl.run_per_block()
+    np.testing.assert_array_equal(bl.collect(), ref)
+    ra, rb = np.nonzero(ba == 0)[0], np.nonzero(bb == 0)[0]
+    _, _, counts = oracle.comparison_fit([dfa[v].to_numpy()[ra] for v in vf], [dfb[v].to_numpy()[rb] for v in vf],
+                                         [dfa["e"].to_numpy()[ra]], [dfb["e"].to_numpy()[rb]])
+    np.testing.assert_array_equal(bl.Block_Counts[0], counts)
+    assert bl.Block_Counts[:, 1:].sum() > 0
 
 
 def test_blocked_linkage_with_long_values(oracle):
