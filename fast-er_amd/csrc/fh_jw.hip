@@ -578,6 +578,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   }
   FH_TRY(launch_build_table(chars_b, offsets_b, ent_b, d_tiles, ntiles, d_small, krows, tw, d_table, s));
   uint32_t* d_codes = nullptr;
+  uint32_t* d_codes_b = nullptr;
   int64_t codes_hi0 = 0, codes_hi_base = 0;
   unsigned long long* d_skipped = nullptr;  // per string-1 length: steps skipped per tile pass (stats)
   if (prefix) {
@@ -588,6 +589,16 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
     FH_TRY(launch_prefix_codes(chars_a, offsets_a, ent_a, c1, c2, d_small, krows, hi0, hi_base, d_codes, s));
     codes_hi0 = hi0;
     codes_hi_base = hi_base;
+    // String 2's codes for the survivor scoring (values of 1..32 bytes; in
+    // deduplication both sides are one sorted set and share codes_a).
+    if (mode == FH_MODE_DEDUP) {
+      d_codes_b = d_codes;
+    } else {
+      const int64_t b1 = start_b[1], b2 = start_b[33];
+      FH_TRY(scr.alloc(&d_codes_b, (size_t)std::max<int64_t>(1, n_b) * kPrefixCodeWords));
+      FH_TRY(launch_prefix_codes(chars_b, offsets_b, ent_b, b1, b2, d_small, krows, b2,
+                                 std::max<int64_t>(1, n_b) * kPrefixCodeWords, d_codes_b, s));
+    }
     if (stats && c2 > c1) {
       FH_TRY(scr.alloc(&d_skipped, kPrefixMaxLen + 1));
       FH_TRY(fill_u64(reinterpret_cast<uint64_t*>(d_skipped), 0, kPrefixMaxLen + 1, s));
@@ -640,6 +651,7 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   args.codes_a = d_codes;
   args.codes_hi0 = codes_hi0;
   args.codes_hi_base = codes_hi_base;
+  args.codes_b = d_codes_b;
 
   // d_args: the arguments of the prefix passes and of the scoring
   // kernel; d_args_lane (no candidate buffer): the per-lane kernel, which

@@ -198,6 +198,9 @@ struct JwArgs {
   int64_t codes_hi0;      // first sorted index of a value over 32 bytes
   int64_t codes_hi_base;  // words 8-15 of those values start at codes_a + codes_hi_base
   const int64_t* cls_start_a;  // [kNumClasses]
+  // Row codes of every string-2 value of at most 32 bytes in its sorted
+  // order, same layout (words 0-7); in deduplication the string-1 codes.
+  const uint32_t* codes_b;
 };
 
 constexpr unsigned long long kEmptyKey = ~0ull;

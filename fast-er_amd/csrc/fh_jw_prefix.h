@@ -505,43 +505,55 @@ __global__ void __launch_bounds__(256) k_prefix_restarts(const uint32_t* __restr
     if (h[i]) atomicAdd(&skipped[i], h[i]);
 }
 
-// Level of string 1 = the value with row codes cw and length la against
-// string 2 = sorted value sb of length lb (both at most 32 bytes): the
-// reference's greedy match, transpositions and prefix
-// (faster/comparison.py:48-123) from string 2's mask-table column and
-// string 1's row codes, with all of string 1's rows read at once into
-// registers (candidate_level reads them again for the transpositions and
-// the prefix).
-__device__ __forceinline__ int table_level(const JwArgs& a, const int64_t* starts_b, const uint32_t (&cw)[8], int la,
-                                           int64_t sb, int lb) {
+// Bit k (k = 0..3) set where byte k of `word` equals the byte that `rep`
+// repeats in all four bytes: bit 7 of each byte of y is set where x's byte is
+// nonzero (the low seven bits carry into it, or it was set). The zero flags
+// of bytes 0-2 are gathered into bits 21-23 by one 24-bit multiply (full
+// rate; its partial products do not overlap), byte 3's flag is bit 31.
+__device__ __forceinline__ uint32_t equal_bytes(uint32_t word, uint32_t rep) {
+  const uint32_t x = word ^ rep;
+  const uint32_t y = ((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x;
+  const uint32_t z = ~y & 0x80808080u;
+  const uint32_t lo3 = (__umul24(z >> 7, 0x00204081u) >> 21) & 0x7u;
+  return lo3 | ((z >> 28) & 0x8u);
+}
+
+// Level of string 1 (row codes c1, length la) against string 2 (row codes
+// c2, length lb), both of at most 32 bytes: the reference's greedy match,
+// transpositions and prefix (faster/comparison.py:48-123) on per-character
+// position masks built in registers: row i = the positions of string 2 whose
+// code equals string 1's code i. Codes are the alphabet map of bytes present
+// on both sides; every other byte and every position past a value's end
+// holds the last code, which matches nothing.
+// `lmax` is wave-uniform and at least la and lb of every lane that calls:
+// the unrolled steps past it are skipped by scalar branches.
+__device__ __forceinline__ int codes_level(const JwArgs& a, const uint32_t (&c1)[8], int la, const uint32_t (&c2)[8],
+                                           int lb, int lmax) {
   const int mm = a.mmin[la * (kLongLen + 1) + lb];
-  const int kind = kind_for(lb, a.p5 != 0);
-  const int64_t pos = sb - starts_b[lb];
-  const int per = kind_tile_strings(kind, a.tw);
-  const int64_t tile = a.cls_tile0[lb] + pos / per;
-  const int p = (int)(pos % per);
-  int col = p, shift = 0;
-  if (packed_kind(kind)) {
-    col = p / kind_lanes(kind);
-    shift = kind_lane_shift(kind, p % kind_lanes(kind));
-  }
-  const uint32_t* base = a.table + (size_t)tile * a.krows * a.tw + col;
+  const uint32_t none = (uint32_t)(a.krows - 1);
   const uint32_t lmask = lb >= 32 ? ~0u : ((1u << lb) - 1u);
-  // Row i: string 2's positions holding string 1's byte i.
+  const int nw2 = (lb + 3) >> 2;
   uint32_t row[32];
 #pragma unroll
   for (int i = 0; i < 32; ++i) {
     row[i] = 0;
-    if (i < la) {
-      const uint32_t code = (cw[i >> 2] >> (8 * (i & 3))) & 0xFFu;
-      row[i] = (~base[(size_t)code * a.tw] >> shift) & lmask;
+    if (i < lmax && i < la) {
+      const uint32_t code = (c1[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+      if (code != none) {
+        const uint32_t rep = __builtin_amdgcn_perm(code, code, 0u);  // byte 0 of code in every byte
+        uint32_t r = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (4 * k < lmax && k < nw2) r |= equal_bytes(c2[k], rep) << (4 * k);
+        row[i] = r & lmask;
+      }
     }
   }
   const int w = max(la, lb) / 2 - 1;
   uint32_t used = 0, flag1 = 0;
 #pragma unroll
   for (int i = 0; i < 32; ++i) {
-    if (i < la) {
+    if (i < lmax && i < la) {
       const uint32_t avail = row[i] & (uint32_t)window_bits(i, w, lb) & ~used;
       if (avail) {
         used |= avail & (~avail + 1u);
@@ -556,7 +568,7 @@ __device__ __forceinline__ int table_level(const JwArgs& a, const int64_t* start
   uint32_t u = used;
 #pragma unroll
   for (int i = 0; i < 32; ++i) {
-    if ((flag1 >> i) & 1u) {
+    if (i < lmax && ((flag1 >> i) & 1u)) {
       const int j = __builtin_ctz(u);
       u &= u - 1u;
       if (!((row[i] >> j) & 1u)) ++tt;
@@ -571,9 +583,19 @@ __device__ __forceinline__ int table_level(const JwArgs& a, const int64_t* start
   return level_of(jw_formula(la, lb, m, tt, pre, a.p), a.lo, a.hi);
 }
 
-// Row codes of sorted value si (string-1 side).
-__device__ __forceinline__ void load_codes(const JwArgs& a, int64_t si, uint32_t (&cw)[8]) {
-  const uint32_t* cp = a.codes_a + (size_t)si * kPrefixCodeWords;
+// Largest v over the wave's lanes for 0 <= v <= 63 (five ballots; every
+// lane of the wave must call it).
+__device__ __forceinline__ int wave_max_small(int v) {
+  int m = 0;
+#pragma unroll
+  for (int b = 32; b >= 1; b >>= 1)
+    if (__ballot(v >= m + b) != 0) m += b;
+  return __builtin_amdgcn_readfirstlane(m);
+}
+
+// Row codes of sorted value si (words 0-7) of a code array.
+__device__ __forceinline__ void load_codes(const uint32_t* codes, int64_t si, uint32_t (&cw)[8]) {
+  const uint32_t* cp = codes + (size_t)si * kPrefixCodeWords;
 #pragma unroll
   for (int k = 0; k < 8; ++k) cw[k] = cp[k];
 }
@@ -604,37 +626,27 @@ __device__ __forceinline__ bool codes_greater(const uint32_t (&c1)[8], int l1, c
 }
 
 // Level of survivor (sa, sb) of the prefix form, string 1 = sa, by
-// table_level; string-1 values over 32 bytes go to candidate_level.
+// codes_level; string-1 values over 32 bytes go to candidate_level.
 // Deduplication pairs whose byte order is reversed are scored with the roles
-// exchanged (string 1 = sb): in deduplication both sides share one sorted
-// order, one code array and one mask table, so sb's row codes and sa's
-// table column exist.
-__device__ __forceinline__ int prefix_level(const JwArgs& a, const int64_t* starts_a, const int64_t* starts_b,
-                                            int64_t sa, int64_t sb, uint64_t& x, uint64_t& y) {
+// exchanged (string 1 = sb), as the reference scores the larger value as
+// string 1.
+// la, lb: the length classes of sa and sb; lmax: wave-uniform, at least
+// la and lb of every lane with la <= 32.
+__device__ __forceinline__ int prefix_level(const JwArgs& a, int64_t sa, int64_t sb, int la, int lb, int lmax,
+                                            uint64_t& x, uint64_t& y) {
   const uint64_t ea = a.sorted_a[sa], eb = a.sorted_b[sb];
   x = ea;
   y = eb;
-  const int la = class_of(starts_a, sa);
-  const int lb = class_of(starts_b, sb);
   if (la > 32) return candidate_level(a, sa, sb, x, y);
-  uint32_t cw[8];
-  load_codes(a, sa, cw);
-  int l1 = la, l2 = lb;
-  int64_t s2 = sb;
-  if (a.mode == FH_MODE_DEDUP) {
-    uint32_t cw2[8];
-    load_codes(a, sb, cw2);
-    if (!codes_greater(cw, la, cw2, lb)) {
-      x = eb;
-      y = ea;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) cw[k] = cw2[k];
-      l1 = lb;
-      l2 = la;
-      s2 = sa;
-    }
+  uint32_t cw[8], cw2[8];
+  load_codes(a.codes_a, sa, cw);
+  load_codes(a.codes_b, sb, cw2);
+  if (a.mode == FH_MODE_DEDUP && !codes_greater(cw, la, cw2, lb)) {
+    x = eb;
+    y = ea;
+    return codes_level(a, cw2, lb, cw, la, lmax);
   }
-  return table_level(a, starts_b, cw, l1, s2, l2);
+  return codes_level(a, cw, la, cw2, lb, lmax);
 }
 
 // Appends the wave's buffered level keys to a.hits with one atomic on the
@@ -653,7 +665,11 @@ __device__ __forceinline__ void eval_flush(const JwArgs& a, const uint64_t* buf,
 // and iteration on that single address serialised the scorer.
 constexpr int kEvalBuf = 256;
 
-__global__ void __launch_bounds__(256) k_eval_prefix(const JwArgs* __restrict__ ap, int64_t n) {
+#ifndef FH_EVAL_WAVES_PER_EU
+#define FH_EVAL_WAVES_PER_EU 8
+#endif
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FH_EVAL_WAVES_PER_EU))) k_eval_prefix(const JwArgs* __restrict__ ap, int64_t n) {
   const JwArgs& a = *ap;
   __shared__ int64_t starts[2][kNumClasses];
   __shared__ uint64_t hbuf[256 / 64][kEvalBuf];
@@ -671,10 +687,19 @@ __global__ void __launch_bounds__(256) k_eval_prefix(const JwArgs* __restrict__ 
     const int64_t e = e0 + threadIdx.x;
     int lvl = 0;
     uint64_t x = 0, y = 0;
+    int64_t sa = 0, sb = 0;
+    int la = 0, lb = 0;
     if (e < n) {
       const uint64_t v = a.cand[e];
-      lvl = prefix_level(a, starts[0], starts[1], (int64_t)(v >> 32), (int64_t)(v & 0xFFFFFFFFull), x, y);
+      sa = (int64_t)(v >> 32);
+      sb = (int64_t)(v & 0xFFFFFFFFull);
+      la = class_of(starts[0], sa);
+      lb = class_of(starts[1], sb);
     }
+    // Longest value of the wave's pairs scored from row codes (string 1 of
+    // at most 32 bytes; string 2 always is).
+    const int lmax = wave_max_small(la <= 32 ? max(la, lb) : 0);
+    if (e < n) lvl = prefix_level(a, sa, sb, la, lb, lmax, x, y);
     if (a.hkeys) {
       emit_level(a, x, y, lvl);
       continue;
