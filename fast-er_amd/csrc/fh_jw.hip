@@ -533,8 +533,9 @@ extern "C" int fh_jw_levels(const uint8_t* chars_a, const int64_t* offsets_a, in
   int64_t cand_cap = (int64_t)1 << 27;
   int64_t* d_cls = nullptr;
   // The prefix form's candidate pairs go to one buffer per round and are
-  // scored by a separate kernel; the per-lane form scores its survivors in
-  // the kernel. Both find a string-2 value's table column from its sorted
+  // scored by a separate kernel from the row codes of both values (the
+  // lengths from the class starts); the per-lane form scores its survivors
+  // in the kernel and finds a string-2 value's table column from its sorted
   // index through the class starts and first tile of each class.
   //
   // Rounds run over string-1 units (waves), each sized so that its

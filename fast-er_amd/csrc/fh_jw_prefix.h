@@ -4,8 +4,9 @@
 // Replaces, like k_jw_tiles, the reference's comparison kernel
 // (faster/comparison.py:133-186, deduplication copy
 // faster/deduplication.py:132-196) and its thresholding
-// (faster/comparison.py:321-338); the match recurrence, mask table, pruning
-// bound and survivor scoring are the ones of fh_jw_core.h.
+// (faster/comparison.py:321-338); the match recurrence, mask table and
+// pruning bound are the ones of fh_jw_core.h, the survivors are scored here
+// (k_eval_prefix) from both values' row codes.
 //
 // Transposed roles. A wave still owns 64 string-1 values of one length LA,
 // but instead of one per lane it walks them one after another, and its lanes
