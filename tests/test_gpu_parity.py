@@ -145,6 +145,21 @@ def test_mask_kind_boundaries(oracle, mode, lens, alpha, n):
         _eq_lists(jaro_winkler_dedup_gpu(S), oracle.jaro_winkler_dedup(S), f"dedup {lens}")
 
 
+def test_bytes_on_one_side_only(oracle):
+    """Bytes that occur on one side only share the alphabet's last code with
+    the positions past a value's end; they must match nothing, in the
+    comparison pass and in the survivor scoring (prefix form: both values'
+    row codes), while the shared bytes still match."""
+    from faster.comparison import jaro_winkler_unique_gpu
+
+    rng = np.random.default_rng(23)
+    A = _words(rng, 1500, "abcdxy", 1, 12)
+    B = _words(rng, 1500, "abcduv", 1, 12)
+    B += [s.replace("x", "u").replace("y", "v") for s in A[:500]]  # near copies across the two alphabets
+    B += [s for s in A[500:700] if "x" not in s and "y" not in s]  # exact copies over shared bytes
+    _eq_lists(jaro_winkler_unique_gpu(A, B), oracle.jaro_winkler_unique(A, B), "one-sided bytes")
+
+
 @pytest.mark.parametrize("n_sym", [60, 120, 0])
 def test_large_alphabets(oracle, n_sym):
     """Alphabets of more byte values than a 256-dword tile holds rows for, so
